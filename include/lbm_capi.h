@@ -1,0 +1,183 @@
+/*
+ * lbm_capi.h -- C ABI of the MI355X (gfx950) LBM collide-stream engine.
+ *
+ * Plain C types only (pointers, sizes, enums); every call returns an
+ * lbm_status and never aborts.  All device work is ordered on the stream of
+ * the handle it is issued on; the host blocks only in calls that return host
+ * data (get_state, moments, mass, run_timed, sync).
+ *
+ * Two families of entry points:
+ *
+ *  1. lbm_sparse_*  -- drop-in replacement of the reference's kernel classes.
+ *     The caller owns a fluid-only, indirectly addressed node list and the
+ *     int32 adjacency that encodes streaming and halfway bounce-back
+ *     (src/geometries/GeometryAAAB.cpp:18-124).  Each call is one launch over
+ *     the fluid-index range [start, start + count), exactly like
+ *       KernelAA::timestepEvenForce / timestepOddForce (src/kernels/KernelAA.hh:28-29)
+ *       KernelAB::timestepForce                        (src/kernels/KernelAB.hh:28)
+ *     Population arrays are device pointers owned by the caller (allocated with
+ *     lbm_device_alloc, as Propagation*::setup allocates its Pdf arrays,
+ *     src/propagation/PropagationAA.cpp:10-25).  Arithmetic is the reference
+ *     Base kernels' (src/kernels/KernelAA_SOA_Base.cpp:185-247), bit for bit in
+ *     FP64 and in FP32 (-D_PRECISION_SINGLE_ semantics).
+ *
+ *  2. lbm_dense_*   -- the MI355X fast path.  The whole domain of one GPU is a
+ *     structured nx*ny*nz box (x fastest); populations are stored SOA as
+ *     wavefront-contiguous x-rows (or AOS); neighbours are found by index
+ *     arithmetic with periodic wrap in-kernel (the single-GPU replacement of the
+ *     MPI halo exchange, src/Communication.cpp:379-481); solid cells and
+ *     non-periodic faces bounce back through a per-cell link mask.  Storage can
+ *     be FP64, FP32 deviations (f - w_s) or scaled FP16 deviations; the FP64
+ *     path is the reference arithmetic bit for bit.  Replaces
+ *     PropagationAA::run / PropagationAB::run (src/propagation/PropagationAA.cpp:126-215,
+ *     src/propagation/PropagationAB.cpp:155-219) together with the kernels they call.
+ *
+ * Canonical population order used at the host boundary (set/get_state):
+ *   f[s * ncell + cell], cell = x + nx * (y + ny * z), s in reference order,
+ *   holding the pre-collision populations f_s(x) at the current time step.
+ */
+#ifndef LBM_CAPI_H
+#define LBM_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  LBM_OK = 0,
+  LBM_ERR_ARG = 1,          /* invalid argument (null pointer, bad size, bad enum) */
+  LBM_ERR_HIP = 2,          /* a HIP runtime call failed */
+  LBM_ERR_OOM = 3,          /* device allocation failed */
+  LBM_ERR_UNSUPPORTED = 4,  /* combination not implemented */
+  LBM_ERR_STATE = 5,        /* call not valid in the handle's current state */
+  LBM_ERR_NUMERIC = 6       /* NaN / Inf detected by a checking reduction */
+} lbm_status;
+
+/* reference propPattern values (src/Parameters.cpp:26, "AB = 0, AA = 1") */
+typedef enum { LBM_PATTERN_AB = 0, LBM_PATTERN_AA = 1 } lbm_pattern;
+/* Makefile:3 -DSOA / -DAOS */
+typedef enum { LBM_LAYOUT_SOA = 0, LBM_LAYOUT_AOS = 1 } lbm_layout;
+typedef enum {
+  LBM_STORE_F64 = 0,   /* double populations (reference Pdf = double) */
+  LBM_STORE_F32S = 1,  /* float deviations f - w_s, FP32 collision, FP64 moment sums */
+  LBM_STORE_F16S = 2,  /* half deviations (f - w_s) * scale, FP32 collision, FP64 moments */
+  LBM_STORE_F32 = 3    /* float populations, reference -D_PRECISION_SINGLE_ arithmetic (sparse) */
+} lbm_storage;
+
+/* Last error message of the calling thread ("" if none). */
+const char* lbm_last_error(void);
+/* Library version string. */
+const char* lbm_version(void);
+
+/* ---------------------------------------------------------------- device memory
+ * Replaces the device allocations and copies of the reference's SYCL/Kokkos
+ * builds (sycl::malloc_device / q_.memcpy, src/propagation/PropagationAA.cpp:17-23,
+ * :117-118, :193-199). */
+lbm_status lbm_device_count(int* count);
+lbm_status lbm_set_device(int device);
+lbm_status lbm_device_alloc(size_t bytes, void** ptr);
+lbm_status lbm_device_free(void* ptr);
+lbm_status lbm_memcpy_h2d(void* dst, const void* src, size_t bytes);
+lbm_status lbm_memcpy_d2h(void* dst, const void* src, size_t bytes);
+lbm_status lbm_device_synchronize(void);
+
+/* ---------------------------------------------------------------- sparse (drop-in) */
+typedef struct lbm_sparse lbm_sparse;
+
+typedef struct {
+  int64_t n_fluid;           /* Geometry::getNumFluidPts (src/geometries/Geometry.hh:68) */
+  int32_t q;                 /* 19 | 27 (_LATTICESIZE_, src/Parameters.hh:6-8) */
+  int32_t layout;            /* lbm_layout */
+  int32_t storage;           /* LBM_STORE_F64 (Pdf = double) | LBM_STORE_F32 (Pdf = float) */
+  int32_t adjacency_mode;    /* 0: neighbour at x + c_s (AB push); 1: x - c_s (AA, AB pull) */
+  double omega;              /* _OMEGA_ (src/Parameters.hh:26-28) */
+  double force[3];           /* (_GRAVITY_, 0, 0) in the reference (src/Parameters.hh:36) */
+  const int32_t* adjacency;  /* host, n_fluid * q entries, Geometry::adjacency_ layout */
+} lbm_sparse_config;
+
+/* Kernel*::setup (src/kernels/KernelAA_SOA_Base.cpp:8-57) plus the device copy of
+ * the adjacency (src/geometries/GeometryAAAB.cpp:115-121). */
+lbm_status lbm_sparse_create(const lbm_sparse_config* cfg, lbm_sparse** out);
+lbm_status lbm_sparse_destroy(lbm_sparse* h);
+/* KernelAA::timestepEvenForce (src/kernels/KernelAA.hh:28): read slot opp(s), write slot s. */
+lbm_status lbm_sparse_aa_even(lbm_sparse* h, void* dstrb, int64_t start, int64_t count);
+/* KernelAA::timestepOddForce (src/kernels/KernelAA.hh:29): gather/scatter through the adjacency. */
+lbm_status lbm_sparse_aa_odd(lbm_sparse* h, void* dstrb, int64_t start, int64_t count);
+/* KernelAB::timestepForce (src/kernels/KernelAB.hh:28); pull = 0 -> KernelAB_SOA_Base /
+ * KernelAB_AOS_Base (scatter), pull = 1 -> KernelAB_SOA_Pull (gather). */
+lbm_status lbm_sparse_ab(lbm_sparse* h, const void* src, void* tgt, int64_t start, int64_t count,
+                         int32_t pull);
+/* PropagationAA::initialize (aa = 1, w_s into slot opp(s)) / PropagationAB::initialize
+ * (aa = 0, w_s into slot s) (src/propagation/PropagationAA.cpp:60-123, PropagationAB.cpp:91-152). */
+lbm_status lbm_sparse_init_rest(lbm_sparse* h, void* dstrb, int32_t aa);
+/* Stream-order point: waits for every launch issued on the handle. */
+lbm_status lbm_sparse_synchronize(lbm_sparse* h);
+
+/* ---------------------------------------------------------------- dense (fast path) */
+typedef struct lbm_dense lbm_dense;
+
+typedef struct {
+  int32_t nx, ny, nz;
+  int32_t q;                 /* 19 | 27 */
+  int32_t pattern;           /* lbm_pattern */
+  int32_t pull;              /* AB only: 0 push, 1 pull (reference deck key `pull`) */
+  int32_t layout;            /* lbm_layout */
+  int32_t storage;           /* LBM_STORE_F64 | LBM_STORE_F32S | LBM_STORE_F16S */
+  int32_t periodic[3];       /* wrap flags; a non-periodic face bounces back */
+  double omega;              /* 1 / tau */
+  double force[3];           /* body force (Guo forcing) */
+  const uint8_t* solid;      /* host, nx*ny*nz, nonzero = solid; NULL = all fluid */
+  const double* tau;         /* host per-cell tau (extension), NULL = uniform omega */
+  const double* gamma;       /* host per-cell Brinkman gamma in [0,1] (extension), NULL = none */
+  double f16_scale;          /* FP16 deviation scale (power of two); 0 -> 4096 */
+} lbm_dense_config;
+
+typedef struct {
+  int64_t ncell;             /* nx * ny * nz */
+  int64_t n_fluid;           /* non-solid cells (MLUPS counts these) */
+  int64_t time_step;         /* completed steps since the last set_state / init */
+  double bytes_per_lup;      /* algorithmic HBM bytes per fluid-node update (2 Q b) */
+  size_t device_bytes;       /* device memory held by the handle */
+} lbm_dense_info;
+
+lbm_status lbm_dense_create(const lbm_dense_config* cfg, lbm_dense** out);
+lbm_status lbm_dense_destroy(lbm_dense* h);
+lbm_status lbm_dense_get_info(lbm_dense* h, lbm_dense_info* info);
+/* Canonical pre-collision state in / out (host arrays of q * ncell doubles);
+ * set_state resets time_step to 0.  Solid cells read back as 0. */
+lbm_status lbm_dense_set_state(lbm_dense* h, const double* f);
+lbm_status lbm_dense_get_state(lbm_dense* h, double* f);
+/* f = f^eq(rho, u) per cell: rho (ncell, NULL -> 1) and u (3 * ncell, xyz per cell)
+ * are host arrays.  Replaces the rest initialisation of PropagationAA/AB::initialize. */
+lbm_status lbm_dense_init_equilibrium(lbm_dense* h, const double* rho, const double* u);
+/* Device-side Taylor-Green initial state: rho = 1,
+ * u = u0 sin(kx x) cos(ky y) cos(kz z), v = -u0 cos(kx x) sin(ky y) cos(kz z), w = 0,
+ * k = 2 pi / n per axis, plus noise * (2 U - 1) per component with
+ * U = top53(splitmix64(seed * 0x9E3779B97F4A7C15 + 3 * cell + component)) / 2^53. */
+lbm_status lbm_dense_init_tgv(lbm_dense* h, double u0, double noise, uint64_t seed);
+/* Advance n steps (asynchronous).  PropagationAA::run / PropagationAB::run loop body. */
+lbm_status lbm_dense_step(lbm_dense* h, int32_t nsteps);
+/* Advance n steps and return their device time (HIP events on the handle's stream). */
+lbm_status lbm_dense_run_timed(lbm_dense* h, int32_t nsteps, double* elapsed_ms);
+/* rho and u = m / rho per cell of the current pre-collision state, host outputs
+ * (ncell and 3 * ncell doubles; solid cells 0).  PropagationAA::getRho / getVelocity
+ * (src/propagation/PropagationAA.cpp:218-327) for all cells at once. */
+lbm_status lbm_dense_moments(lbm_dense* h, double* rho, double* u);
+/* FP64 sum of all fluid populations (deterministic two-pass reduction);
+ * LBM_ERR_NUMERIC if the sum is not finite. */
+lbm_status lbm_dense_mass(lbm_dense* h, double* mass);
+lbm_status lbm_dense_synchronize(lbm_dense* h);
+
+/* ---------------------------------------------------------------- measurement helper */
+/* Device-to-device copy kernel of `bytes` (multiple of 16) between two buffers
+ * allocated by the library; used to calibrate achieved-bandwidth and
+ * HBM-counter readings.  Returns the time of `reps` launches in ms. */
+lbm_status lbm_copy_bandwidth(size_t bytes, int32_t reps, double* elapsed_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LBM_CAPI_H */

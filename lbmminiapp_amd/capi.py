@@ -1,0 +1,301 @@
+"""ctypes binding of the C ABI declared in include/lbm_capi.h.
+
+The HIP library must be built (``python -m lbmminiapp_amd.build``); there is no
+CPU fallback: every class here raises if liblbm_hip.so is missing or a call
+fails.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "liblbm_hip.so")
+
+# enums (include/lbm_capi.h)
+OK, ERR_ARG, ERR_HIP, ERR_OOM, ERR_UNSUPPORTED, ERR_STATE, ERR_NUMERIC = range(7)
+PATTERN_AB, PATTERN_AA = 0, 1
+LAYOUT_SOA, LAYOUT_AOS = 0, 1
+STORE_F64, STORE_F32S, STORE_F16S, STORE_F32 = 0, 1, 2, 3
+
+STORAGE_NAMES = {"f64": STORE_F64, "f32s": STORE_F32S, "f16s": STORE_F16S, "f32": STORE_F32}
+LAYOUT_NAMES = {"soa": LAYOUT_SOA, "aos": LAYOUT_AOS}
+
+EXPORTED = [
+    "lbm_last_error", "lbm_version", "lbm_device_count", "lbm_set_device", "lbm_device_alloc",
+    "lbm_device_free", "lbm_memcpy_h2d", "lbm_memcpy_d2h", "lbm_device_synchronize",
+    "lbm_sparse_create", "lbm_sparse_destroy", "lbm_sparse_aa_even", "lbm_sparse_aa_odd",
+    "lbm_sparse_ab", "lbm_sparse_init_rest", "lbm_sparse_synchronize",
+    "lbm_dense_create", "lbm_dense_destroy", "lbm_dense_get_info", "lbm_dense_set_state",
+    "lbm_dense_get_state", "lbm_dense_init_equilibrium", "lbm_dense_init_tgv", "lbm_dense_step",
+    "lbm_dense_run_timed", "lbm_dense_moments", "lbm_dense_mass", "lbm_dense_synchronize",
+    "lbm_copy_bandwidth",
+]
+
+
+class LbmError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"lbm status {status}: {msg}")
+        self.status = status
+
+
+class _SparseConfig(ctypes.Structure):
+    _fields_ = [("n_fluid", ctypes.c_int64), ("q", ctypes.c_int32), ("layout", ctypes.c_int32),
+                ("storage", ctypes.c_int32), ("adjacency_mode", ctypes.c_int32),
+                ("omega", ctypes.c_double), ("force", ctypes.c_double * 3),
+                ("adjacency", ctypes.c_void_p)]
+
+
+class _DenseConfig(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("q", ctypes.c_int32), ("pattern", ctypes.c_int32), ("pull", ctypes.c_int32),
+                ("layout", ctypes.c_int32), ("storage", ctypes.c_int32),
+                ("periodic", ctypes.c_int32 * 3), ("omega", ctypes.c_double),
+                ("force", ctypes.c_double * 3), ("solid", ctypes.c_void_p),
+                ("tau", ctypes.c_void_p), ("gamma", ctypes.c_void_p),
+                ("f16_scale", ctypes.c_double)]
+
+
+class _DenseInfo(ctypes.Structure):
+    _fields_ = [("ncell", ctypes.c_int64), ("n_fluid", ctypes.c_int64),
+                ("time_step", ctypes.c_int64), ("bytes_per_lup", ctypes.c_double),
+                ("device_bytes", ctypes.c_size_t)]
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Loads liblbm_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} not found: build it with `python -m lbmminiapp_amd.build`")
+    L = ctypes.CDLL(path)
+    P, I32, I64, D = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+    L.lbm_last_error.restype = ctypes.c_char_p
+    L.lbm_version.restype = ctypes.c_char_p
+    sig = {
+        "lbm_device_count": [P], "lbm_set_device": [ctypes.c_int],
+        "lbm_device_alloc": [ctypes.c_size_t, P], "lbm_device_free": [P],
+        "lbm_memcpy_h2d": [P, P, ctypes.c_size_t], "lbm_memcpy_d2h": [P, P, ctypes.c_size_t],
+        "lbm_device_synchronize": [],
+        "lbm_sparse_create": [P, P], "lbm_sparse_destroy": [P],
+        "lbm_sparse_aa_even": [P, P, I64, I64], "lbm_sparse_aa_odd": [P, P, I64, I64],
+        "lbm_sparse_ab": [P, P, P, I64, I64, I32], "lbm_sparse_init_rest": [P, P, I32],
+        "lbm_sparse_synchronize": [P],
+        "lbm_dense_create": [P, P], "lbm_dense_destroy": [P], "lbm_dense_get_info": [P, P],
+        "lbm_dense_set_state": [P, P], "lbm_dense_get_state": [P, P],
+        "lbm_dense_init_equilibrium": [P, P, P], "lbm_dense_init_tgv": [P, D, D, ctypes.c_uint64],
+        "lbm_dense_step": [P, I32], "lbm_dense_run_timed": [P, I32, P],
+        "lbm_dense_moments": [P, P, P], "lbm_dense_mass": [P, P], "lbm_dense_synchronize": [P],
+        "lbm_copy_bandwidth": [ctypes.c_size_t, I32, P],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def check(status: int) -> None:
+    if status != OK:
+        msg = load().lbm_last_error().decode(errors="replace")
+        raise LbmError(status, msg)
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    check(load().lbm_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def copy_bandwidth(nbytes: int, reps: int = 10) -> float:
+    """Achieved copy bandwidth in GB/s (read + write bytes / time)."""
+    ms = ctypes.c_double(0)
+    check(load().lbm_copy_bandwidth(nbytes, reps, ctypes.byref(ms)))
+    return 2.0 * nbytes * reps / (ms.value * 1e-3) / 1e9
+
+
+# --------------------------------------------------------------------------- device buffers
+class DeviceBuffer:
+    """A device allocation owned by the library (lbm_device_alloc)."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        self.ptr = ctypes.c_void_p()
+        check(load().lbm_device_alloc(self.nbytes, ctypes.byref(self.ptr)))
+
+    @classmethod
+    def from_array(cls, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        check(load().lbm_memcpy_h2d(b.ptr, _p(a), a.nbytes))
+        return b
+
+    def to_array(self, dtype, count=None) -> np.ndarray:
+        dtype = np.dtype(dtype)
+        count = self.nbytes // dtype.itemsize if count is None else count
+        out = np.empty(count, dtype)
+        check(load().lbm_memcpy_d2h(_p(out), self.ptr, out.nbytes))
+        return out
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        check(load().lbm_memcpy_h2d(self.ptr, _p(a), a.nbytes))
+
+    def free(self):
+        if self.ptr:
+            check(load().lbm_device_free(self.ptr))
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# --------------------------------------------------------------------------- sparse
+class SparseKernels:
+    """Drop-in kernels on the reference's fluid list + adjacency (lbm_sparse_*)."""
+
+    def __init__(self, n_fluid, q, adjacency, omega, force=(1e-6, 0.0, 0.0), layout="soa",
+                 storage="f64", adjacency_mode=1):
+        self.q = q
+        self.n = int(n_fluid)
+        self.storage = STORAGE_NAMES[storage] if isinstance(storage, str) else storage
+        self.dtype = np.float64 if self.storage == STORE_F64 else np.float32
+        self._adj = np.ascontiguousarray(adjacency, np.int32)
+        cfg = _SparseConfig(self.n, q, LAYOUT_NAMES[layout] if isinstance(layout, str) else layout,
+                            self.storage, adjacency_mode, float(omega),
+                            (ctypes.c_double * 3)(*force), _p(self._adj))
+        self.h = ctypes.c_void_p()
+        check(load().lbm_sparse_create(ctypes.byref(cfg), ctypes.byref(self.h)))
+
+    def aa_even(self, buf: DeviceBuffer, start=0, count=None):
+        check(load().lbm_sparse_aa_even(self.h, buf.ptr, start, self.n if count is None else count))
+
+    def aa_odd(self, buf: DeviceBuffer, start=0, count=None):
+        check(load().lbm_sparse_aa_odd(self.h, buf.ptr, start, self.n if count is None else count))
+
+    def ab(self, src: DeviceBuffer, tgt: DeviceBuffer, start=0, count=None, pull=False):
+        check(load().lbm_sparse_ab(self.h, src.ptr, tgt.ptr, start,
+                                   self.n if count is None else count, int(pull)))
+
+    def init_rest(self, buf: DeviceBuffer, aa: bool):
+        check(load().lbm_sparse_init_rest(self.h, buf.ptr, int(aa)))
+
+    def synchronize(self):
+        check(load().lbm_sparse_synchronize(self.h))
+
+    def close(self):
+        if self.h:
+            check(load().lbm_sparse_destroy(self.h))
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# --------------------------------------------------------------------------- dense
+class DenseLattice:
+    """Structured-grid engine (lbm_dense_*)."""
+
+    def __init__(self, nx, ny, nz, q=19, pattern="aa", pull=False, layout="soa", storage="f64",
+                 periodic=(1, 1, 1), omega=1.0, force=(0.0, 0.0, 0.0), solid=None, tau=None,
+                 gamma=None, f16_scale=0.0):
+        self.shape = (int(nx), int(ny), int(nz))
+        self.q = q
+        self.ncell = int(nx) * int(ny) * int(nz)
+        pat = {"aa": PATTERN_AA, "ab": PATTERN_AB}[pattern] if isinstance(pattern, str) else pattern
+        self._solid = None if solid is None else np.ascontiguousarray(solid, np.uint8).reshape(-1)
+        self._tau = None if tau is None else np.ascontiguousarray(tau, np.float64).reshape(-1)
+        self._gamma = None if gamma is None else np.ascontiguousarray(gamma, np.float64).reshape(-1)
+        for a in (self._solid, self._tau, self._gamma):
+            if a is not None and a.size != self.ncell:
+                raise ValueError("per-cell arrays must have nx*ny*nz entries")
+        cfg = _DenseConfig(nx, ny, nz, q, pat, int(pull),
+                           LAYOUT_NAMES[layout] if isinstance(layout, str) else layout,
+                           STORAGE_NAMES[storage] if isinstance(storage, str) else storage,
+                           (ctypes.c_int32 * 3)(*[int(p) for p in periodic]), float(omega),
+                           (ctypes.c_double * 3)(*force), _p(self._solid), _p(self._tau),
+                           _p(self._gamma), float(f16_scale))
+        self.h = ctypes.c_void_p()
+        check(load().lbm_dense_create(ctypes.byref(cfg), ctypes.byref(self.h)))
+
+    def info(self) -> dict:
+        i = _DenseInfo()
+        check(load().lbm_dense_get_info(self.h, ctypes.byref(i)))
+        return {"ncell": i.ncell, "n_fluid": i.n_fluid, "time_step": i.time_step,
+                "bytes_per_lup": i.bytes_per_lup, "device_bytes": i.device_bytes}
+
+    def set_state(self, f: np.ndarray):
+        f = np.ascontiguousarray(f, np.float64).reshape(-1)
+        assert f.size == self.q * self.ncell
+        check(load().lbm_dense_set_state(self.h, _p(f)))
+
+    def get_state(self) -> np.ndarray:
+        out = np.empty(self.q * self.ncell, np.float64)
+        check(load().lbm_dense_get_state(self.h, _p(out)))
+        return out.reshape(self.q, self.ncell)
+
+    def init_equilibrium(self, u: np.ndarray, rho: np.ndarray | None = None):
+        u = np.ascontiguousarray(u, np.float64).reshape(-1)
+        assert u.size == 3 * self.ncell
+        r = None if rho is None else np.ascontiguousarray(rho, np.float64).reshape(-1)
+        check(load().lbm_dense_init_equilibrium(self.h, _p(r), _p(u)))
+
+    def init_tgv(self, u0=0.05, noise=1e-4, seed=1):
+        check(load().lbm_dense_init_tgv(self.h, float(u0), float(noise), int(seed)))
+
+    def step(self, n=1):
+        check(load().lbm_dense_step(self.h, int(n)))
+
+    def run_timed(self, n) -> float:
+        ms = ctypes.c_double(0)
+        check(load().lbm_dense_run_timed(self.h, int(n), ctypes.byref(ms)))
+        return ms.value
+
+    def moments(self):
+        rho = np.empty(self.ncell)
+        u = np.empty(3 * self.ncell)
+        check(load().lbm_dense_moments(self.h, _p(rho), _p(u)))
+        return rho, u.reshape(self.ncell, 3)
+
+    def mass(self) -> float:
+        m = ctypes.c_double(0)
+        check(load().lbm_dense_mass(self.h, ctypes.byref(m)))
+        return m.value
+
+    def synchronize(self):
+        check(load().lbm_dense_synchronize(self.h))
+
+    def close(self):
+        if self.h:
+            check(load().lbm_dense_destroy(self.h))
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,680 @@
+// lbm_dense.hip -- host side of the structured-grid engine: handle, link-mask
+// builder, state initialisation and readout, moments, FP64 mass reduction,
+// and the time loop (the replacement of PropagationAA::run / PropagationAB::run,
+// src/propagation/PropagationAA.cpp:126-215, src/propagation/PropagationAB.cpp:155-219).
+//
+// State conventions (pre-collision populations f_s(x) at time t):
+//   AA, t even      f_s(x) in slot opp(s) at x (src/propagation/PropagationAA.cpp:111)
+//   AA, t odd       f_s(x) in slot s at x - c_s, or slot opp(s) at x when x - c_s is
+//                   solid (src/propagation/PropagationAA.cpp:243-257)
+//   AB push         f_s(x) in slot s at x of the current array
+//   AB pull         the current array holds post-collision values; f_s(x) is gathered
+//                   like the AA odd case (src/kernels/KernelAB_SOA_Pull.cpp:139-140)
+#include <cmath>
+#include <vector>
+
+#include "lbm_dense_kernels.hpp"
+
+namespace lbm {
+
+enum CanonMode { C_LOCAL = 0, C_OPP = 1, C_GATHER = 2 };
+
+// storage index of f_s(x) for a given state convention
+template <int Q, int LAYOUT>
+__device__ __forceinline__ int64_t canon_index(int mode, int s, int64_t cell, uint32_t lk,
+                                               const Offsets& off, int64_t n) {
+  const int o = Lattice<Q>::opp(s);
+  if (mode == C_LOCAL) return slot<Q, LAYOUT>(n, s, cell);
+  if (mode == C_OPP) return slot<Q, LAYOUT>(n, o, cell);
+  if ((lk >> o) & 1u) return slot<Q, LAYOUT>(n, o, cell);
+  return slot<Q, LAYOUT>(n, s,
+                         cell + off.at(-Lattice<Q>::c(s, 0), -Lattice<Q>::c(s, 1),
+                                       -Lattice<Q>::c(s, 2)));
+}
+
+__device__ __forceinline__ bool cell_coords(const DenseArgs& a, int& x, int& y, int& z,
+                                            int64_t& cell) {
+  x = blockIdx.x * blockDim.x + threadIdx.x;
+  y = blockIdx.y;
+  z = blockIdx.z;
+  if (x >= a.nx) return false;
+  cell = x + (int64_t)a.nx * (y + (int64_t)a.ny * z);
+  return true;
+}
+
+// ---------------------------------------------------------------- link mask
+template <int Q>
+__global__ void __launch_bounds__(256)
+    k_build_links(const uint8_t* __restrict__ solid, uint32_t* __restrict__ links, int nx, int ny,
+                  int nz, int px, int py, int pz) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y, z = blockIdx.z;
+  if (x >= nx) return;
+  const int64_t cell = x + (int64_t)nx * (y + (int64_t)ny * z);
+  if (solid[cell]) {
+    links[cell] = SOLID_BIT;
+    return;
+  }
+  uint32_t m = 0;
+  for (int s = 0; s < Q; s++) {
+    int p[3] = {x + Lattice<Q>::c(s, 0), y + Lattice<Q>::c(s, 1), z + Lattice<Q>::c(s, 2)};
+    const int dims[3] = {nx, ny, nz};
+    const int per[3] = {px, py, pz};
+    bool wall = false;
+    for (int k = 0; k < 3; k++) {
+      if (p[k] < 0 || p[k] >= dims[k]) {
+        if (!per[k]) wall = true;
+        else p[k] = p[k] < 0 ? p[k] + dims[k] : p[k] - dims[k];
+      }
+    }
+    if (!wall) wall = solid[p[0] + (int64_t)nx * (p[1] + (int64_t)ny * p[2])] != 0;
+    if (wall) m |= 1u << s;
+  }
+  links[cell] = m;
+}
+
+// ---------------------------------------------------------------- state in / out
+// f_s(x) -> storage, for every fluid cell; src is canonical [Q][ncell] (device).
+template <int Q, int STORE, int LAYOUT>
+__global__ void __launch_bounds__(256)
+    k_set_canonical(const double* __restrict__ src, void* fv, DenseArgs a, int mode) {
+  auto* f = static_cast<typename Storage<STORE>::T*>(fv);
+  int x, y, z;
+  int64_t cell;
+  if (!cell_coords(a, x, y, z, cell)) return;
+  const uint32_t lk = a.links ? a.links[cell] : 0u;
+  if (lk & SOLID_BIT) return;
+  const Offsets off(x, y, z, a.nx, a.ny, a.nz);
+  for (int s = 0; s < Q; s++) {
+    double v = src[a.ncell * s + cell];
+    if constexpr (STORE != LBM_STORE_F64) v -= Lattice<Q>::w(s);
+    st<STORE>(f, canon_index<Q, LAYOUT>(mode, s, cell, lk, off, a.ncell),
+              (typename Storage<STORE>::C)v, a.scale);
+  }
+}
+
+template <int STORE>
+__device__ __forceinline__ double to_f(typename Storage<STORE>::C v, double w) {
+  if constexpr (STORE == LBM_STORE_F64) return v;
+  else return w + (double)v;
+}
+
+template <int Q, int STORE, int LAYOUT>
+__global__ void __launch_bounds__(256)
+    k_get_canonical(const void* fv, double* __restrict__ dst, DenseArgs a, int mode) {
+  const auto* f = static_cast<const typename Storage<STORE>::T*>(fv);
+  int x, y, z;
+  int64_t cell;
+  if (!cell_coords(a, x, y, z, cell)) return;
+  const uint32_t lk = a.links ? a.links[cell] : 0u;
+  const Offsets off(x, y, z, a.nx, a.ny, a.nz);
+  for (int s = 0; s < Q; s++) {
+    double v = 0.0;
+    if (!(lk & SOLID_BIT))
+      v = to_f<STORE>(ld<STORE>(f, canon_index<Q, LAYOUT>(mode, s, cell, lk, off, a.ncell),
+                                a.inv_scale),
+                      Lattice<Q>::w(s));
+    dst[a.ncell * s + cell] = v;
+  }
+}
+
+// rho, u = m / rho of the pre-collision state (reference readout: no F/2 shift,
+// src/propagation/PropagationAA.cpp:260-327).
+template <int Q, int STORE, int LAYOUT>
+__global__ void __launch_bounds__(256)
+    k_moments(const void* fv, double* __restrict__ rho_out, double* __restrict__ u_out,
+              DenseArgs a, int mode) {
+  const auto* f = static_cast<const typename Storage<STORE>::T*>(fv);
+  int x, y, z;
+  int64_t cell;
+  if (!cell_coords(a, x, y, z, cell)) return;
+  const uint32_t lk = a.links ? a.links[cell] : 0u;
+  double rho = 0.0, m0 = 0.0, m1 = 0.0, m2 = 0.0;
+  if (!(lk & SOLID_BIT)) {
+    const Offsets off(x, y, z, a.nx, a.ny, a.nz);
+    for (int s = 0; s < Q; s++) {
+      const double v =
+          to_f<STORE>(ld<STORE>(f, canon_index<Q, LAYOUT>(mode, s, cell, lk, off, a.ncell),
+                                a.inv_scale),
+                      Lattice<Q>::w(s));
+      rho += v;
+      m0 += v * Lattice<Q>::c(s, 0);
+      m1 += v * Lattice<Q>::c(s, 1);
+      m2 += v * Lattice<Q>::c(s, 2);
+    }
+    const double inv = 1.0 / rho;
+    m0 *= inv;
+    m1 *= inv;
+    m2 *= inv;
+  }
+  rho_out[cell] = rho;
+  u_out[3 * cell + 0] = m0;
+  u_out[3 * cell + 1] = m1;
+  u_out[3 * cell + 2] = m2;
+}
+
+// ---------------------------------------------------------------- initial states
+__device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ double uniform01(uint64_t seed, uint64_t counter) {
+  const uint64_t key = seed * 0x9E3779B97F4A7C15ull + counter;
+  return (double)(splitmix64(key) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// src_kind 0: rest; 1: rho/u device arrays; 2: Taylor-Green vortex + seeded noise.
+template <int Q, int STORE, int LAYOUT>
+__global__ void __launch_bounds__(256)
+    k_init(void* fv, DenseArgs a, int mode, int src_kind,
+           const double* __restrict__ rho_in, const double* __restrict__ u_in, double u0,
+           double noise, uint64_t seed) {
+  auto* f = static_cast<typename Storage<STORE>::T*>(fv);
+  int x, y, z;
+  int64_t cell;
+  if (!cell_coords(a, x, y, z, cell)) return;
+  const uint32_t lk = a.links ? a.links[cell] : 0u;
+  if (lk & SOLID_BIT) return;
+  double rho = 1.0, u[3] = {0.0, 0.0, 0.0};
+  if (src_kind == 1) {
+    if (rho_in) rho = rho_in[cell];
+    for (int k = 0; k < 3; k++) u[k] = u_in[3 * cell + k];
+  } else if (src_kind == 2) {
+    const double kx = 2.0 * M_PI / a.nx, ky = 2.0 * M_PI / a.ny, kz = 2.0 * M_PI / a.nz;
+    u[0] = u0 * sin(kx * x) * cos(ky * y) * cos(kz * z);
+    u[1] = -u0 * cos(kx * x) * sin(ky * y) * cos(kz * z);
+    if (noise != 0.0)
+      for (int k = 0; k < 3; k++)
+        u[k] += noise * (2.0 * uniform01(seed, 3ull * (uint64_t)cell + k) - 1.0);
+  }
+  const Offsets off(x, y, z, a.nx, a.ny, a.nz);
+  // f^eq with the collision's association (src/kernels/KernelAA_SOA_Base.cpp:227-238, F = 0)
+  const double vdotv = 3.0 * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+  for (int s = 0; s < Q; s++) {
+    const double c0 = Lattice<Q>::c(s, 0), c1 = Lattice<Q>::c(s, 1), c2 = Lattice<Q>::c(s, 2);
+    const double cdotv = 3.0 * (c0 * u[0] + c1 * u[1] + c2 * u[2]);
+    const double w = Lattice<Q>::w(s);
+    double feq = w * rho * (1.0 + cdotv + 0.5 * cdotv * cdotv - 0.5 * vdotv);
+    if constexpr (STORE != LBM_STORE_F64) feq = w * ((rho - 1.0) + rho * (cdotv + 0.5 * cdotv * cdotv - 0.5 * vdotv));
+    st<STORE>(f, canon_index<Q, LAYOUT>(mode, s, cell, lk, off, a.ncell),
+              (typename Storage<STORE>::C)feq, a.scale);
+  }
+}
+
+// ---------------------------------------------------------------- mass
+// Sum over every fluid cell of its Q stored values (any state convention is a
+// permutation of the canonical populations over the fluid slots).
+template <int Q, int STORE, int LAYOUT>
+__global__ void __launch_bounds__(256)
+    k_mass_partial(const void* fv, DenseArgs a, double* partial) {
+  const auto* f = static_cast<const typename Storage<STORE>::T*>(fv);
+  __shared__ double red[256];
+  double acc = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; cell < a.ncell;
+       cell += stride) {
+    if (a.links && (a.links[cell] & SOLID_BIT)) continue;
+    for (int s = 0; s < Q; s++)
+      acc += to_f<STORE>(ld<STORE>(f, slot<Q, LAYOUT>(a.ncell, s, cell), a.inv_scale),
+                         Lattice<Q>::w(s));
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(256) k_sum_partials(const double* partial, int n, double* out) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) acc += partial[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+// ---------------------------------------------------------------- dispatch helpers
+template <template <int, int, int> class K, typename... A>
+void launch_util(int q, int storage, int layout, dim3 g, dim3 b, hipStream_t st, A... args) {
+#define LBM_UTIL_CASE(QQ, SS, LL) \
+  if (q == QQ && storage == SS && layout == LL) return K<QQ, SS, LL>::launch(g, b, st, args...);
+#define LBM_UTIL_Q(QQ)                                      \
+  LBM_UTIL_CASE(QQ, LBM_STORE_F64, LBM_LAYOUT_SOA)          \
+  LBM_UTIL_CASE(QQ, LBM_STORE_F32S, LBM_LAYOUT_SOA)         \
+  LBM_UTIL_CASE(QQ, LBM_STORE_F16S, LBM_LAYOUT_SOA)         \
+  LBM_UTIL_CASE(QQ, LBM_STORE_F64, LBM_LAYOUT_AOS)          \
+  LBM_UTIL_CASE(QQ, LBM_STORE_F32S, LBM_LAYOUT_AOS)         \
+  LBM_UTIL_CASE(QQ, LBM_STORE_F16S, LBM_LAYOUT_AOS)
+  LBM_UTIL_Q(19)
+  LBM_UTIL_Q(27)
+#undef LBM_UTIL_Q
+#undef LBM_UTIL_CASE
+}
+
+#define LBM_UTIL_KERNEL(name)                                                             \
+  template <int Q, int S, int L>                                                          \
+  struct name##_u {                                                                       \
+    template <typename... A>                                                              \
+    static void launch(dim3 g, dim3 b, hipStream_t st, A... a) {                          \
+      hipLaunchKernelGGL((name<Q, S, L>), g, b, 0, st, a...);                             \
+    }                                                                                     \
+  };
+LBM_UTIL_KERNEL(k_set_canonical)
+LBM_UTIL_KERNEL(k_get_canonical)
+LBM_UTIL_KERNEL(k_moments)
+LBM_UTIL_KERNEL(k_init)
+LBM_UTIL_KERNEL(k_mass_partial)
+#undef LBM_UTIL_KERNEL
+
+}  // namespace lbm
+
+using namespace lbm;
+
+struct lbm_dense {
+  int nx = 0, ny = 0, nz = 0, q = 19, pattern = 1, pull = 0, layout = 0, storage = 0;
+  int periodic[3] = {1, 1, 1};
+  int64_t ncell = 0, nfluid = 0, t = 0;
+  size_t esize = 8;
+  void* f[2] = {nullptr, nullptr};
+  int cur = 0;
+  uint32_t* links = nullptr;
+  double* omega_field = nullptr;
+  double* gamma_field = nullptr;
+  double* partial = nullptr;  // mass partial sums + result
+  int npartial = 1024;
+  hipStream_t stream = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  DenseArgs args{};
+  size_t device_bytes = 0;
+};
+
+namespace {
+
+dim3 block_for(int nx) {
+  int bx = nx >= 256 ? 256 : ((nx + 63) / 64) * 64;
+  return dim3(bx);
+}
+
+dim3 grid_for(const lbm_dense* h) {
+  const dim3 b = block_for(h->nx);
+  return dim3((h->nx + b.x - 1) / b.x, h->ny, h->nz);
+}
+
+// state convention of the current array
+int canon_mode(const lbm_dense* h) {
+  if (h->pattern == LBM_PATTERN_AA) return (h->t % 2 == 0) ? C_OPP : C_GATHER;
+  return h->pull ? C_GATHER : C_LOCAL;
+}
+
+DenseLaunchFn pick_step(const lbm_dense* h, int kind) {
+  const bool mask = h->links != nullptr;
+  return h->q == 19 ? pick_dense19(h->storage, h->layout, kind, mask)
+                    : pick_dense27(h->storage, h->layout, kind, mask);
+}
+
+lbm_status enqueue_steps(lbm_dense* h, int nsteps) {
+  const dim3 g = grid_for(h), b = block_for(h->nx);
+  for (int i = 0; i < nsteps; i++) {
+    if (h->pattern == LBM_PATTERN_AA) {
+      const int kind = (h->t % 2 == 0) ? K_AA_EVEN : K_AA_ODD;
+      DenseLaunchFn fn = pick_step(h, kind);
+      LBM_REQUIRE(fn, LBM_ERR_UNSUPPORTED, "no kernel for this configuration");
+      fn(g, b, h->stream, h->f[0], h->f[0], h->args);
+    } else {
+      DenseLaunchFn fn = pick_step(h, h->pull ? K_AB_PULL : K_AB_PUSH);
+      LBM_REQUIRE(fn, LBM_ERR_UNSUPPORTED, "no kernel for this configuration");
+      fn(g, b, h->stream, h->f[h->cur], h->f[1 - h->cur], h->args);
+      h->cur = 1 - h->cur;
+    }
+    LBM_LAUNCH_CHECK();
+    h->t++;
+  }
+  return LBM_OK;
+}
+
+void free_all(lbm_dense* h) {
+  if (!h) return;
+  for (int i = 0; i < 2; i++)
+    if (h->f[i]) (void)hipFree(h->f[i]);
+  if (h->links) (void)hipFree(h->links);
+  if (h->omega_field) (void)hipFree(h->omega_field);
+  if (h->gamma_field) (void)hipFree(h->gamma_field);
+  if (h->partial) (void)hipFree(h->partial);
+  if (h->e0) (void)hipEventDestroy(h->e0);
+  if (h->e1) (void)hipEventDestroy(h->e1);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+lbm_status dmalloc(lbm_dense* h, void** p, size_t bytes, const char* what) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    set_error("lbm_dense_create: hipMalloc of %zu bytes for %s failed: %s", bytes, what,
+              hipGetErrorString(e));
+    return LBM_ERR_OOM;
+  }
+  h->device_bytes += bytes;
+  return LBM_OK;
+}
+
+lbm_status init_from(lbm_dense* h, int src_kind, const double* rho_d, const double* u_d,
+                     double u0, double noise, uint64_t seed) {
+  h->t = 0;
+  h->cur = 0;
+  const int mode = canon_mode(h);
+  launch_util<k_init_u>(h->q, h->storage, h->layout, grid_for(h), block_for(h->nx), h->stream,
+                        h->f[0], h->args, mode, src_kind, rho_d, u_d, u0, noise, seed);
+  LBM_LAUNCH_CHECK();
+  return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+lbm_status lbm_dense_create(const lbm_dense_config* cfg, lbm_dense** out) {
+  LBM_REQUIRE(cfg && out, LBM_ERR_ARG, "lbm_dense_create: null pointer");
+  *out = nullptr;
+  LBM_REQUIRE(cfg->nx > 0 && cfg->ny > 0 && cfg->nz > 0, LBM_ERR_ARG,
+              "lbm_dense_create: dimensions must be positive (got %d x %d x %d)", cfg->nx,
+              cfg->ny, cfg->nz);
+  LBM_REQUIRE(cfg->ny <= 65535 && cfg->nz <= 65535, LBM_ERR_ARG,
+              "lbm_dense_create: ny and nz must be <= 65535");
+  LBM_REQUIRE(cfg->q == 19 || cfg->q == 27, LBM_ERR_ARG, "lbm_dense_create: q must be 19 or 27");
+  LBM_REQUIRE(cfg->pattern == LBM_PATTERN_AA || cfg->pattern == LBM_PATTERN_AB, LBM_ERR_ARG,
+              "lbm_dense_create: bad pattern %d", cfg->pattern);
+  LBM_REQUIRE(cfg->layout == LBM_LAYOUT_SOA || cfg->layout == LBM_LAYOUT_AOS, LBM_ERR_ARG,
+              "lbm_dense_create: bad layout %d", cfg->layout);
+  LBM_REQUIRE(cfg->storage == LBM_STORE_F64 || cfg->storage == LBM_STORE_F32S ||
+                  cfg->storage == LBM_STORE_F16S,
+              LBM_ERR_ARG, "lbm_dense_create: storage must be F64, F32S or F16S");
+  LBM_REQUIRE(std::isfinite(cfg->omega) && cfg->omega > 0.0 && cfg->omega < 2.0, LBM_ERR_ARG,
+              "lbm_dense_create: omega must be in (0, 2)");
+  lbm_dense* h = new lbm_dense;
+  h->nx = cfg->nx;
+  h->ny = cfg->ny;
+  h->nz = cfg->nz;
+  h->q = cfg->q;
+  h->pattern = cfg->pattern;
+  h->pull = cfg->pattern == LBM_PATTERN_AB ? (cfg->pull ? 1 : 0) : 0;
+  h->layout = cfg->layout;
+  h->storage = cfg->storage;
+  for (int k = 0; k < 3; k++) h->periodic[k] = cfg->periodic[k] ? 1 : 0;
+  h->ncell = (int64_t)cfg->nx * cfg->ny * cfg->nz;
+  h->esize = cfg->storage == LBM_STORE_F64 ? 8 : (cfg->storage == LBM_STORE_F32S ? 4 : 2);
+  // fluid count (host mask)
+  h->nfluid = h->ncell;
+  if (cfg->solid) {
+    int64_t ns = 0;
+    for (int64_t c = 0; c < h->ncell; c++) ns += cfg->solid[c] != 0;
+    h->nfluid = h->ncell - ns;
+  }
+  const bool need_links =
+      cfg->solid != nullptr || !h->periodic[0] || !h->periodic[1] || !h->periodic[2];
+  lbm_status st = LBM_OK;
+  auto fail = [&](lbm_status s) {
+    free_all(h);
+    return s;
+  };
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&h->e0) != hipSuccess || hipEventCreate(&h->e1) != hipSuccess) {
+    set_error("lbm_dense_create: stream/event creation failed");
+    return fail(LBM_ERR_HIP);
+  }
+  const size_t fbytes = (size_t)h->ncell * h->q * h->esize;
+  const int narrays = h->pattern == LBM_PATTERN_AA ? 1 : 2;
+  for (int i = 0; i < narrays; i++)
+    if ((st = dmalloc(h, &h->f[i], fbytes, "populations"))) return fail(st);
+  if ((st = dmalloc(h, (void**)&h->partial, sizeof(double) * (h->npartial + 1), "reduction")))
+    return fail(st);
+  if (need_links) {
+    if ((st = dmalloc(h, (void**)&h->links, sizeof(uint32_t) * h->ncell, "link mask")))
+      return fail(st);
+    uint8_t* solid_d = nullptr;
+    if (hipMalloc(&solid_d, h->ncell) != hipSuccess) {
+      set_error("lbm_dense_create: hipMalloc of the solid mask failed");
+      return fail(LBM_ERR_OOM);
+    }
+    hipError_t e = cfg->solid
+                       ? hipMemcpyAsync(solid_d, cfg->solid, h->ncell, hipMemcpyHostToDevice,
+                                        h->stream)
+                       : hipMemsetAsync(solid_d, 0, h->ncell, h->stream);
+    if (e == hipSuccess) {
+      const dim3 g = grid_for(h), b = block_for(h->nx);
+      if (h->q == 19)
+        hipLaunchKernelGGL(k_build_links<19>, g, b, 0, h->stream, solid_d, h->links, h->nx,
+                           h->ny, h->nz, h->periodic[0], h->periodic[1], h->periodic[2]);
+      else
+        hipLaunchKernelGGL(k_build_links<27>, g, b, 0, h->stream, solid_d, h->links, h->nx,
+                           h->ny, h->nz, h->periodic[0], h->periodic[1], h->periodic[2]);
+      e = hipGetLastError();
+      if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    }
+    (void)hipFree(solid_d);
+    if (e != hipSuccess) {
+      set_error("lbm_dense_create: link mask build failed: %s", hipGetErrorString(e));
+      return fail(LBM_ERR_HIP);
+    }
+  }
+  if (cfg->tau) {
+    std::vector<double> om(h->ncell);
+    for (int64_t c = 0; c < h->ncell; c++) {
+      const double tau = cfg->tau[c];
+      const bool is_solid = cfg->solid && cfg->solid[c];
+      if (!is_solid && !(std::isfinite(tau) && tau > 0.5)) {
+        set_error("lbm_dense_create: tau[%lld] = %g must be > 0.5", (long long)c, tau);
+        return fail(LBM_ERR_ARG);
+      }
+      om[c] = is_solid ? 1.0 : 1.0 / tau;
+    }
+    if ((st = dmalloc(h, (void**)&h->omega_field, sizeof(double) * h->ncell, "omega field")))
+      return fail(st);
+    if (hipMemcpy(h->omega_field, om.data(), sizeof(double) * h->ncell, hipMemcpyHostToDevice) !=
+        hipSuccess) {
+      set_error("lbm_dense_create: omega field copy failed");
+      return fail(LBM_ERR_HIP);
+    }
+  }
+  if (cfg->gamma) {
+    if ((st = dmalloc(h, (void**)&h->gamma_field, sizeof(double) * h->ncell, "gamma field")))
+      return fail(st);
+    if (hipMemcpy(h->gamma_field, cfg->gamma, sizeof(double) * h->ncell,
+                  hipMemcpyHostToDevice) != hipSuccess) {
+      set_error("lbm_dense_create: gamma field copy failed");
+      return fail(LBM_ERR_HIP);
+    }
+  }
+  const double scale = cfg->f16_scale > 0.0 ? cfg->f16_scale : 4096.0;
+  DenseArgs& a = h->args;
+  a.nx = h->nx;
+  a.ny = h->ny;
+  a.nz = h->nz;
+  a.ncell = h->ncell;
+  a.links = h->links;
+  a.omega_field = h->omega_field;
+  a.gamma_field = h->gamma_field;
+  a.omega = cfg->omega;
+  a.F0 = cfg->force[0];
+  a.F1 = cfg->force[1];
+  a.F2 = cfg->force[2];
+  a.scale = (float)scale;
+  a.inv_scale = (float)(1.0 / scale);
+  // rest state, as PropagationAA/AB::initialize (src/propagation/PropagationAA.cpp:60-123)
+  if ((st = init_from(h, 0, nullptr, nullptr, 0.0, 0.0, 0))) return fail(st);
+  if (h->pattern == LBM_PATTERN_AB &&
+      hipMemcpyAsync(h->f[1], h->f[0], fbytes, hipMemcpyDeviceToDevice, h->stream) != hipSuccess) {
+    set_error("lbm_dense_create: initial copy failed");
+    return fail(LBM_ERR_HIP);
+  }
+  if (hipStreamSynchronize(h->stream) != hipSuccess) {
+    set_error("lbm_dense_create: initialisation failed");
+    return fail(LBM_ERR_HIP);
+  }
+  *out = h;
+  return LBM_OK;
+}
+
+lbm_status lbm_dense_destroy(lbm_dense* h) {
+  if (!h) return LBM_OK;
+  (void)hipStreamSynchronize(h->stream);
+  free_all(h);
+  return LBM_OK;
+}
+
+lbm_status lbm_dense_get_info(lbm_dense* h, lbm_dense_info* info) {
+  LBM_REQUIRE(h && info, LBM_ERR_ARG, "lbm_dense_get_info: null pointer");
+  info->ncell = h->ncell;
+  info->n_fluid = h->nfluid;
+  info->time_step = h->t;
+  info->bytes_per_lup = 2.0 * h->q * (double)h->esize;
+  info->device_bytes = h->device_bytes;
+  return LBM_OK;
+}
+
+lbm_status lbm_dense_set_state(lbm_dense* h, const double* f) {
+  LBM_REQUIRE(h && f, LBM_ERR_ARG, "lbm_dense_set_state: null pointer");
+  const size_t bytes = sizeof(double) * h->q * h->ncell;
+  double* stage = nullptr;
+  LBM_HIP_CHECK(hipMalloc(&stage, bytes));
+  hipError_t e = hipMemcpyAsync(stage, f, bytes, hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess) {
+    h->t = 0;
+    h->cur = 0;
+    launch_util<k_set_canonical_u>(h->q, h->storage, h->layout, grid_for(h), block_for(h->nx),
+                                   h->stream, (const double*)stage, h->f[0], h->args,
+                                   canon_mode(h));
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  }
+  (void)hipFree(stage);
+  if (e != hipSuccess) {
+    set_error("lbm_dense_set_state: %s", hipGetErrorString(e));
+    return LBM_ERR_HIP;
+  }
+  return LBM_OK;
+}
+
+lbm_status lbm_dense_get_state(lbm_dense* h, double* f) {
+  LBM_REQUIRE(h && f, LBM_ERR_ARG, "lbm_dense_get_state: null pointer");
+  const size_t bytes = sizeof(double) * h->q * h->ncell;
+  double* stage = nullptr;
+  LBM_HIP_CHECK(hipMalloc(&stage, bytes));
+  launch_util<k_get_canonical_u>(h->q, h->storage, h->layout, grid_for(h), block_for(h->nx),
+                                 h->stream, (const void*)h->f[h->cur], stage, h->args,
+                                 canon_mode(h));
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(f, stage, bytes, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  (void)hipFree(stage);
+  if (e != hipSuccess) {
+    set_error("lbm_dense_get_state: %s", hipGetErrorString(e));
+    return LBM_ERR_HIP;
+  }
+  return LBM_OK;
+}
+
+lbm_status lbm_dense_init_equilibrium(lbm_dense* h, const double* rho, const double* u) {
+  LBM_REQUIRE(h && u, LBM_ERR_ARG, "lbm_dense_init_equilibrium: null pointer");
+  double *rho_d = nullptr, *u_d = nullptr;
+  LBM_HIP_CHECK(hipMalloc(&u_d, sizeof(double) * 3 * h->ncell));
+  hipError_t e = hipMemcpyAsync(u_d, u, sizeof(double) * 3 * h->ncell, hipMemcpyHostToDevice,
+                                h->stream);
+  if (e == hipSuccess && rho) {
+    e = hipMalloc(&rho_d, sizeof(double) * h->ncell);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(rho_d, rho, sizeof(double) * h->ncell, hipMemcpyHostToDevice,
+                         h->stream);
+  }
+  lbm_status st = LBM_OK;
+  if (e == hipSuccess) st = init_from(h, 1, rho_d, u_d, 0.0, 0.0, 0);
+  if (e == hipSuccess && st == LBM_OK) e = hipStreamSynchronize(h->stream);
+  (void)hipFree(u_d);
+  if (rho_d) (void)hipFree(rho_d);
+  if (st != LBM_OK) return st;
+  if (e != hipSuccess) {
+    set_error("lbm_dense_init_equilibrium: %s", hipGetErrorString(e));
+    return LBM_ERR_HIP;
+  }
+  return LBM_OK;
+}
+
+lbm_status lbm_dense_init_tgv(lbm_dense* h, double u0, double noise, uint64_t seed) {
+  LBM_REQUIRE(h, LBM_ERR_ARG, "lbm_dense_init_tgv: null handle");
+  if (lbm_status st = init_from(h, 2, nullptr, nullptr, u0, noise, seed)) return st;
+  LBM_HIP_CHECK(hipStreamSynchronize(h->stream));
+  return LBM_OK;
+}
+
+lbm_status lbm_dense_step(lbm_dense* h, int32_t nsteps) {
+  LBM_REQUIRE(h && nsteps >= 0, LBM_ERR_ARG, "lbm_dense_step: bad arguments");
+  return enqueue_steps(h, nsteps);
+}
+
+lbm_status lbm_dense_run_timed(lbm_dense* h, int32_t nsteps, double* elapsed_ms) {
+  LBM_REQUIRE(h && elapsed_ms && nsteps >= 0, LBM_ERR_ARG, "lbm_dense_run_timed: bad arguments");
+  LBM_HIP_CHECK(hipEventRecord(h->e0, h->stream));
+  if (lbm_status st = enqueue_steps(h, nsteps)) return st;
+  LBM_HIP_CHECK(hipEventRecord(h->e1, h->stream));
+  LBM_HIP_CHECK(hipEventSynchronize(h->e1));
+  float ms = 0.f;
+  LBM_HIP_CHECK(hipEventElapsedTime(&ms, h->e0, h->e1));
+  *elapsed_ms = ms;
+  return LBM_OK;
+}
+
+lbm_status lbm_dense_moments(lbm_dense* h, double* rho, double* u) {
+  LBM_REQUIRE(h && rho && u, LBM_ERR_ARG, "lbm_dense_moments: null pointer");
+  double* buf = nullptr;
+  LBM_HIP_CHECK(hipMalloc(&buf, sizeof(double) * 4 * h->ncell));
+  launch_util<k_moments_u>(h->q, h->storage, h->layout, grid_for(h), block_for(h->nx), h->stream,
+                           (const void*)h->f[h->cur], buf, buf + h->ncell, h->args,
+                           canon_mode(h));
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(rho, buf, sizeof(double) * h->ncell, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(u, buf + h->ncell, sizeof(double) * 3 * h->ncell, hipMemcpyDeviceToHost,
+                       h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  (void)hipFree(buf);
+  if (e != hipSuccess) {
+    set_error("lbm_dense_moments: %s", hipGetErrorString(e));
+    return LBM_ERR_HIP;
+  }
+  return LBM_OK;
+}
+
+lbm_status lbm_dense_mass(lbm_dense* h, double* mass) {
+  LBM_REQUIRE(h && mass, LBM_ERR_ARG, "lbm_dense_mass: null pointer");
+  launch_util<k_mass_partial_u>(h->q, h->storage, h->layout, dim3(h->npartial), dim3(256),
+                                h->stream, (const void*)h->f[h->cur], h->args, h->partial);
+  LBM_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, h->stream, (const double*)h->partial,
+                     h->npartial, h->partial + h->npartial);
+  LBM_LAUNCH_CHECK();
+  LBM_HIP_CHECK(hipMemcpyAsync(mass, h->partial + h->npartial, sizeof(double),
+                               hipMemcpyDeviceToHost, h->stream));
+  LBM_HIP_CHECK(hipStreamSynchronize(h->stream));
+  LBM_REQUIRE(std::isfinite(*mass), LBM_ERR_NUMERIC, "lbm_dense_mass: mass is not finite (%g)",
+              *mass);
+  return LBM_OK;
+}
+
+lbm_status lbm_dense_synchronize(lbm_dense* h) {
+  LBM_REQUIRE(h, LBM_ERR_ARG, "lbm_dense_synchronize: null handle");
+  LBM_HIP_CHECK(hipStreamSynchronize(h->stream));
+  return LBM_OK;
+}
+
+}  // extern "C"

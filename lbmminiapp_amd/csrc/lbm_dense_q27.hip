@@ -1,0 +1,9 @@
+// lbm_dense_q27.hip -- instantiates the D3Q27 structured-grid kernels
+// (one translation unit per velocity set so they compile in parallel).
+#include "lbm_dense_kernels.hpp"
+
+namespace lbm {
+DenseLaunchFn pick_dense27(int storage, int layout, int kind, bool mask) {
+  return pick_dense_q<27>(storage, layout, kind, mask);
+}
+}  // namespace lbm

@@ -1,0 +1,297 @@
+// lbm_sparse.hip -- drop-in GPU replacement of the reference's kernel classes on
+// its fluid-only, indirectly addressed node list.
+//
+// One work-item per fluid node, exactly the reference's parallel_for shape
+// (src/kernels/KernelAA_SOA_Base.cpp:180, :263; KernelAB_SOA_Base.cpp:116;
+// KernelAB_SOA_Pull.cpp:118), with the node's Q populations held in registers
+// and the collision fully unrolled at compile time.  The arithmetic is
+// collide_ref (the Base kernels' association and order, no FMA contraction),
+// so FP64 results and -D_PRECISION_SINGLE_ FP32 results equal the reference's
+// bit for bit.
+//
+// Launches go to the null stream so that they are ordered with the caller's
+// blocking hipMemcpy traffic, as the reference's kernel calls are ordered with
+// its host code.
+#include "lbm_collide.hpp"
+#include "lbm_internal.hpp"
+
+namespace lbm {
+
+template <int Q, int LAYOUT>
+__device__ __forceinline__ int64_t pidx(int64_t n, int s, int64_t i) {
+  return LAYOUT == LBM_LAYOUT_SOA ? n * s + i : (int64_t)Q * i + s;
+}
+
+// AA even: src/kernels/KernelAA_SOA_Base.cpp:173-254 (AOS: KernelAA_AOS_Base.cpp:174-254)
+template <int Q, typename T, int LAYOUT>
+__global__ void __launch_bounds__(256)
+    k_sparse_aa_even(T* __restrict__ f, int64_t n, int64_t start, int64_t count, double omega,
+                     T F0, T F1, T F2) {
+  const int64_t i = start + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= start + count) return;
+  T d[Q];
+#pragma unroll
+  for (int s = 0; s < Q; s++) d[s] = f[pidx<Q, LAYOUT>(n, Lattice<Q>::opp(s), i)];
+  const T F[3] = {F0, F1, F2};
+  collide_ref<Q, T, false>(d, d, omega, F, 0.0);
+#pragma unroll
+  for (int s = 0; s < Q; s++) f[pidx<Q, LAYOUT>(n, s, i)] = d[s];
+}
+
+// AA odd: src/kernels/KernelAA_SOA_Base.cpp:257-342 (AOS: KernelAA_AOS_Base.cpp:258-342)
+template <int Q, typename T, int LAYOUT>
+__global__ void __launch_bounds__(256)
+    k_sparse_aa_odd(T* __restrict__ f, const int32_t* __restrict__ adj, int64_t n, int64_t start,
+                    int64_t count, double omega, T F0, T F1, T F2) {
+  const int64_t i = start + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= start + count) return;
+  T d[Q];
+  int32_t loc[Q];
+#pragma unroll
+  for (int s = 0; s < Q; s++) {
+    loc[s] = adj[pidx<Q, LAYOUT>(n, s, i)];
+    d[s] = f[loc[s]];
+  }
+  const T F[3] = {F0, F1, F2};
+  collide_ref<Q, T, false>(d, d, omega, F, 0.0);
+#pragma unroll
+  for (int s = 0; s < Q; s++) f[loc[Lattice<Q>::opp(s)]] = d[s];
+}
+
+// AB push: src/kernels/KernelAB_SOA_Base.cpp:111-186 (AOS: KernelAB_AOS_Base.cpp:108-182)
+template <int Q, typename T, int LAYOUT>
+__global__ void __launch_bounds__(256)
+    k_sparse_ab_push(const T* __restrict__ src, T* __restrict__ tgt,
+                     const int32_t* __restrict__ adj, int64_t n, int64_t start, int64_t count,
+                     double omega, T F0, T F1, T F2) {
+  const int64_t i = start + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= start + count) return;
+  T d[Q];
+#pragma unroll
+  for (int s = 0; s < Q; s++) d[s] = src[pidx<Q, LAYOUT>(n, s, i)];
+  const T F[3] = {F0, F1, F2};
+  collide_ref<Q, T, false>(d, d, omega, F, 0.0);
+#pragma unroll
+  for (int s = 0; s < Q; s++) tgt[adj[pidx<Q, LAYOUT>(n, s, i)]] = d[s];
+}
+
+// AB pull: src/kernels/KernelAB_SOA_Pull.cpp:113-194
+template <int Q, typename T, int LAYOUT>
+__global__ void __launch_bounds__(256)
+    k_sparse_ab_pull(const T* __restrict__ src, T* __restrict__ tgt,
+                     const int32_t* __restrict__ adj, int64_t n, int64_t start, int64_t count,
+                     double omega, T F0, T F1, T F2) {
+  const int64_t i = start + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= start + count) return;
+  T d[Q];
+#pragma unroll
+  for (int s = 0; s < Q; s++) d[s] = src[adj[pidx<Q, LAYOUT>(n, s, i)]];
+  const T F[3] = {F0, F1, F2};
+  collide_ref<Q, T, false>(d, d, omega, F, 0.0);
+#pragma unroll
+  for (int s = 0; s < Q; s++) tgt[pidx<Q, LAYOUT>(n, s, i)] = d[s];
+}
+
+// PropagationAA::initialize (slot opp(s) := w_s) and PropagationAB::initialize (slot s := w_s).
+template <int Q, typename T, int LAYOUT>
+__global__ void __launch_bounds__(256) k_sparse_init_rest(T* __restrict__ f, int64_t n, int aa) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int s = 0; s < Q; s++) {
+    const int slot = aa ? Lattice<Q>::opp(s) : s;
+    f[pidx<Q, LAYOUT>(n, slot, i)] = T(Lattice<Q>::w(s));
+  }
+}
+
+}  // namespace lbm
+
+struct lbm_sparse {
+  int64_t n = 0;
+  int q = 19;
+  int layout = LBM_LAYOUT_SOA;
+  int storage = LBM_STORE_F64;
+  int adjacency_mode = 1;
+  double omega = 1.0;
+  double force[3] = {1e-6, 0.0, 0.0};
+  int32_t* adj = nullptr;  // device copy of the adjacency
+};
+
+namespace {
+
+using namespace lbm;
+
+inline unsigned grid_for(int64_t count) { return (unsigned)((count + 255) / 256); }
+
+#define LBM_SPARSE_KERNEL_FN(name)                                                         \
+  template <int Q, typename T, int L>                                                      \
+  struct name##_fn {                                                                       \
+    template <typename... A>                                                               \
+    static void launch(unsigned grid, A... a) {                                            \
+      hipLaunchKernelGGL((lbm::name<Q, T, L>), dim3(grid), dim3(256), 0, 0, a...);        \
+    }                                                                                      \
+  };
+LBM_SPARSE_KERNEL_FN(k_sparse_aa_even)
+LBM_SPARSE_KERNEL_FN(k_sparse_aa_odd)
+LBM_SPARSE_KERNEL_FN(k_sparse_ab_push)
+LBM_SPARSE_KERNEL_FN(k_sparse_ab_pull)
+LBM_SPARSE_KERNEL_FN(k_sparse_init_rest)
+#undef LBM_SPARSE_KERNEL_FN
+
+template <template <int, typename, int> class K, typename T, typename... A>
+void launch_qt(const lbm_sparse* h, unsigned grid, A... a) {
+  if (h->q == 19) {
+    if (h->layout == LBM_LAYOUT_SOA) K<19, T, LBM_LAYOUT_SOA>::launch(grid, a...);
+    else K<19, T, LBM_LAYOUT_AOS>::launch(grid, a...);
+  } else {
+    if (h->layout == LBM_LAYOUT_SOA) K<27, T, LBM_LAYOUT_SOA>::launch(grid, a...);
+    else K<27, T, LBM_LAYOUT_AOS>::launch(grid, a...);
+  }
+}
+
+lbm_status check_range(const lbm_sparse* h, int64_t start, int64_t count) {
+  LBM_REQUIRE(start >= 0 && count >= 0 && start + count <= h->n, LBM_ERR_ARG,
+              "fluid range [%lld, %lld) outside [0, %lld)", (long long)start,
+              (long long)(start + count), (long long)h->n);
+  return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+lbm_status lbm_sparse_create(const lbm_sparse_config* cfg, lbm_sparse** out) {
+  LBM_REQUIRE(cfg && out, LBM_ERR_ARG, "lbm_sparse_create: null pointer");
+  *out = nullptr;
+  LBM_REQUIRE(cfg->q == 19 || cfg->q == 27, LBM_ERR_ARG, "lbm_sparse_create: q must be 19 or 27");
+  LBM_REQUIRE(cfg->layout == LBM_LAYOUT_SOA || cfg->layout == LBM_LAYOUT_AOS, LBM_ERR_ARG,
+              "lbm_sparse_create: bad layout %d", cfg->layout);
+  LBM_REQUIRE(cfg->storage == LBM_STORE_F64 || cfg->storage == LBM_STORE_F32, LBM_ERR_ARG,
+              "lbm_sparse_create: storage must be LBM_STORE_F64 or LBM_STORE_F32");
+  LBM_REQUIRE(cfg->n_fluid > 0 && cfg->adjacency, LBM_ERR_ARG,
+              "lbm_sparse_create: need n_fluid > 0 and an adjacency");
+  LBM_REQUIRE(cfg->n_fluid * cfg->q < ((int64_t)1 << 31), LBM_ERR_ARG,
+              "lbm_sparse_create: n_fluid * q must fit the int32 adjacency");
+  const int64_t nq = cfg->n_fluid * cfg->q;
+  for (int64_t k = 0; k < nq; k++)
+    LBM_REQUIRE(cfg->adjacency[k] >= 0 && cfg->adjacency[k] < nq, LBM_ERR_ARG,
+                "lbm_sparse_create: adjacency[%lld] = %d outside [0, %lld)", (long long)k,
+                cfg->adjacency[k], (long long)nq);
+  lbm_sparse* h = new lbm_sparse;
+  h->n = cfg->n_fluid;
+  h->q = cfg->q;
+  h->layout = cfg->layout;
+  h->storage = cfg->storage;
+  h->adjacency_mode = cfg->adjacency_mode;
+  h->omega = cfg->omega;
+  for (int d = 0; d < 3; d++) h->force[d] = cfg->force[d];
+  if (hipMalloc(&h->adj, sizeof(int32_t) * nq) != hipSuccess) {
+    delete h;
+    set_error("lbm_sparse_create: hipMalloc of the adjacency failed");
+    return LBM_ERR_OOM;
+  }
+  hipError_t e = hipMemcpy(h->adj, cfg->adjacency, sizeof(int32_t) * nq, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(h->adj);
+    delete h;
+    set_error("lbm_sparse_create: adjacency copy failed: %s", hipGetErrorString(e));
+    return LBM_ERR_HIP;
+  }
+  *out = h;
+  return LBM_OK;
+}
+
+lbm_status lbm_sparse_destroy(lbm_sparse* h) {
+  if (!h) return LBM_OK;
+  if (h->adj) LBM_HIP_CHECK(hipFree(h->adj));
+  delete h;
+  return LBM_OK;
+}
+
+lbm_status lbm_sparse_aa_even(lbm_sparse* h, void* dstrb, int64_t start, int64_t count) {
+  LBM_REQUIRE(h && dstrb, LBM_ERR_ARG, "lbm_sparse_aa_even: null pointer");
+  if (lbm_status st = check_range(h, start, count)) return st;
+  if (count == 0) return LBM_OK;
+  const unsigned g = grid_for(count);
+  if (h->storage == LBM_STORE_F64)
+    launch_qt<k_sparse_aa_even_fn, double>(h, g, (double*)dstrb, h->n, start, count, h->omega,
+                                          h->force[0], h->force[1], h->force[2]);
+  else
+    launch_qt<k_sparse_aa_even_fn, float>(h, g, (float*)dstrb, h->n, start, count, h->omega,
+                                         (float)h->force[0], (float)h->force[1],
+                                         (float)h->force[2]);
+  LBM_LAUNCH_CHECK();
+  return LBM_OK;
+}
+
+lbm_status lbm_sparse_aa_odd(lbm_sparse* h, void* dstrb, int64_t start, int64_t count) {
+  LBM_REQUIRE(h && dstrb, LBM_ERR_ARG, "lbm_sparse_aa_odd: null pointer");
+  LBM_REQUIRE(h->adjacency_mode == 1, LBM_ERR_STATE,
+              "lbm_sparse_aa_odd: AA needs the x - c_s adjacency (adjacency_mode 1)");
+  if (lbm_status st = check_range(h, start, count)) return st;
+  if (count == 0) return LBM_OK;
+  const unsigned g = grid_for(count);
+  if (h->storage == LBM_STORE_F64)
+    launch_qt<k_sparse_aa_odd_fn, double>(h, g, (double*)dstrb, (const int32_t*)h->adj, h->n,
+                                         start, count, h->omega, h->force[0], h->force[1],
+                                         h->force[2]);
+  else
+    launch_qt<k_sparse_aa_odd_fn, float>(h, g, (float*)dstrb, (const int32_t*)h->adj, h->n, start,
+                                        count, h->omega, (float)h->force[0], (float)h->force[1],
+                                        (float)h->force[2]);
+  LBM_LAUNCH_CHECK();
+  return LBM_OK;
+}
+
+lbm_status lbm_sparse_ab(lbm_sparse* h, const void* src, void* tgt, int64_t start, int64_t count,
+                         int32_t pull) {
+  LBM_REQUIRE(h && src && tgt, LBM_ERR_ARG, "lbm_sparse_ab: null pointer");
+  LBM_REQUIRE(src != tgt, LBM_ERR_ARG, "lbm_sparse_ab: source and target must differ");
+  LBM_REQUIRE(h->adjacency_mode == (pull ? 1 : 0), LBM_ERR_STATE,
+              "lbm_sparse_ab: %s needs adjacency_mode %d", pull ? "pull" : "push", pull ? 1 : 0);
+  LBM_REQUIRE(!pull || h->layout == LBM_LAYOUT_SOA, LBM_ERR_UNSUPPORTED,
+              "lbm_sparse_ab: the reference has no AOS pull kernel");
+  if (lbm_status st = check_range(h, start, count)) return st;
+  if (count == 0) return LBM_OK;
+  const unsigned g = grid_for(count);
+  const int32_t* adj = h->adj;
+  if (h->storage == LBM_STORE_F64) {
+    if (pull)
+      launch_qt<k_sparse_ab_pull_fn, double>(h, g, (const double*)src, (double*)tgt, adj, h->n,
+                                            start, count, h->omega, h->force[0], h->force[1],
+                                            h->force[2]);
+    else
+      launch_qt<k_sparse_ab_push_fn, double>(h, g, (const double*)src, (double*)tgt, adj, h->n,
+                                            start, count, h->omega, h->force[0], h->force[1],
+                                            h->force[2]);
+  } else {
+    const float F0 = (float)h->force[0], F1 = (float)h->force[1], F2 = (float)h->force[2];
+    if (pull)
+      launch_qt<k_sparse_ab_pull_fn, float>(h, g, (const float*)src, (float*)tgt, adj, h->n,
+                                           start, count, h->omega, F0, F1, F2);
+    else
+      launch_qt<k_sparse_ab_push_fn, float>(h, g, (const float*)src, (float*)tgt, adj, h->n,
+                                           start, count, h->omega, F0, F1, F2);
+  }
+  LBM_LAUNCH_CHECK();
+  return LBM_OK;
+}
+
+lbm_status lbm_sparse_init_rest(lbm_sparse* h, void* dstrb, int32_t aa) {
+  LBM_REQUIRE(h && dstrb, LBM_ERR_ARG, "lbm_sparse_init_rest: null pointer");
+  const unsigned g = grid_for(h->n);
+  if (h->storage == LBM_STORE_F64)
+    launch_qt<k_sparse_init_rest_fn, double>(h, g, (double*)dstrb, h->n, (int)aa);
+  else
+    launch_qt<k_sparse_init_rest_fn, float>(h, g, (float*)dstrb, h->n, (int)aa);
+  LBM_LAUNCH_CHECK();
+  return LBM_OK;
+}
+
+lbm_status lbm_sparse_synchronize(lbm_sparse* h) {
+  LBM_REQUIRE(h, LBM_ERR_ARG, "lbm_sparse_synchronize: null handle");
+  LBM_HIP_CHECK(hipStreamSynchronize(0));
+  return LBM_OK;
+}
+
+}  // extern "C"

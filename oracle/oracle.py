@@ -1,0 +1,347 @@
+"""ctypes front end of the CPU oracle (liblbm_oracle.so) and of the reference
+kernels built into oracle/_ref/ (libref_*.so).
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, ``__graft_entry__.smoke()`` and
+``bench.py``'s cpu_baseline leg as the checker; the product package
+(``lbmminiapp_amd``) never imports it.
+
+The C restatement follows the reference file:line ranges listed in
+``lbm_oracle.h``.  This module adds geometry helpers (dense masks, fluid
+orderings, adjacency via ``orc_build_adjacency`` = ``GeometryAAAB::setupAdjacency``,
+src/geometries/GeometryAAAB.cpp:18-124) and numpy restatements of the
+initial-state formulas used by the parity tests.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblbm_oracle.so")
+REF_DIR = os.path.join(HERE, "_ref")
+
+SOA, AOS = 0, 1
+ADJ_PUSH, ADJ_PULL = 0, 1
+
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_I = ctypes.c_int
+_D = ctypes.c_double
+
+
+def build(quiet: bool = True) -> None:
+    """Compile liblbm_oracle.so (and oracle/_ref when the reference is present)."""
+    out = subprocess.run(["make", "-C", HERE, "-j8"], capture_output=True, text=True)
+    if out.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + out.stdout + out.stderr)
+    if not quiet:
+        print(out.stdout)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        for suf in ("f64", "f32"):
+            getattr(L, f"orc_aa_even_{suf}").argtypes = [_I, _I, _I64, _P, _I64, _I64, _D, _P, _P, _P]
+            getattr(L, f"orc_aa_odd_{suf}").argtypes = [_I, _I, _I64, _P, _P, _I64, _I64, _D, _P, _P, _P]
+            getattr(L, f"orc_ab_push_{suf}").argtypes = [_I, _I, _I64, _P, _P, _P, _I64, _I64, _D, _P, _P, _P]
+            getattr(L, f"orc_ab_pull_{suf}").argtypes = [_I, _I, _I64, _P, _P, _P, _I64, _I64, _D, _P, _P, _P]
+            getattr(L, f"orc_init_rest_aa_{suf}").argtypes = [_I, _I, _I64, _P]
+            getattr(L, f"orc_init_rest_ab_{suf}").argtypes = [_I, _I, _I64, _P]
+            getattr(L, f"orc_ab_push_unroll19_{suf}").argtypes = [_I64, _P, _P, _P, _I64, _I64, _D]
+            getattr(L, f"orc_ab_pull_unroll19_{suf}").argtypes = [_I64, _P, _P, _P, _I64, _I64, _D]
+            getattr(L, f"orc_aa_even_unroll19_{suf}").argtypes = [_I64, _P, _I64, _I64, _D]
+            getattr(L, f"orc_aa_odd_unroll19_{suf}").argtypes = [_I64, _P, _P, _I64, _I64, _D]
+        L.orc_lattice.argtypes = [_I, _P, _P, _P]
+        L.orc_build_adjacency.argtypes = [_I, _I, _I, _I, _I, _I, _P, _P, _P, _I64, _P]
+        L.orc_equilibrium.argtypes = [_I, _D, _P, _P]
+        L.orc_objective.argtypes = [_I, _I, _I64, _P, _P, _I, _I]
+        L.orc_objective.restype = _D
+        L.orc_adjoint.argtypes = [_I, _I64, _P, _P, _I, _D, _P, _P, _P, _I, _P, _I, _I, _P, _P, _P, _P]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+# --------------------------------------------------------------------------- lattice
+def lattice(q: int):
+    """(dirs[q,3] int32, opp[q] int32, w[q] float64) in the reference order
+    (src/lattices/LatticeD3Q19.cpp:12-86, LatticeD3Q27.cpp:12-118)."""
+    dirs = np.zeros((q, 3), np.int32)
+    opp = np.zeros(q, np.int32)
+    w = np.zeros(q, np.float64)
+    if lib().orc_lattice(q, _ptr(dirs), _ptr(opp), _ptr(w)) != 0:
+        raise ValueError(f"unsupported lattice Q={q}")
+    return dirs, opp, w
+
+
+# --------------------------------------------------------------------------- seeded RNG
+_GOLD = np.uint64(0x9E3779B97F4A7C15)
+
+
+def splitmix64(x: np.ndarray) -> np.ndarray:
+    """SplitMix64 finaliser (Steele, Lea, Flood 2014), vectorised, wrapping uint64."""
+    with np.errstate(over="ignore"):
+        z = np.asarray(x, dtype=np.uint64) + _GOLD
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def uniform01(seed: int, counters: np.ndarray) -> np.ndarray:
+    """Counter-based U[0,1): top 53 bits of splitmix64(seed * golden + counter)."""
+    with np.errstate(over="ignore"):
+        key = np.uint64(seed) * _GOLD + np.asarray(counters, dtype=np.uint64)
+    return (splitmix64(key) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+# --------------------------------------------------------------------------- geometry
+class DenseGeometry:
+    """A dense nx*ny*nz box with an optional solid mask (cell = x + nx*(y + ny*z)).
+
+    order="x_fastest" numbers fluid nodes in cell order; order="reference"
+    numbers them x-major with z fastest, the reference's bulk ordering
+    (src/geometries/GeometryAAAB.cpp:243-246, Geometry.cpp:113-131)."""
+
+    def __init__(self, nx, ny, nz, solid=None, periodic=(1, 1, 1), order="x_fastest"):
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.ncell = self.nx * self.ny * self.nz
+        if solid is None:
+            solid = np.zeros(self.ncell, np.uint8)
+        self.solid = np.ascontiguousarray(solid, dtype=np.uint8).reshape(-1)
+        assert self.solid.size == self.ncell
+        self.periodic = np.array(periodic, np.int32)
+        fluid_cells = np.nonzero(self.solid == 0)[0].astype(np.int64)
+        if order == "reference":
+            x = fluid_cells % self.nx
+            y = (fluid_cells // self.nx) % self.ny
+            z = fluid_cells // (self.nx * self.ny)
+            key = (x * self.ny + y) * self.nz + z
+            fluid_cells = fluid_cells[np.argsort(key, kind="stable")]
+        self.cell_of_fluid = fluid_cells.astype(np.int32)
+        self.nfluid = int(fluid_cells.size)
+        self.fluid_of_cell = np.full(self.ncell, -1, np.int32)
+        self.fluid_of_cell[self.cell_of_fluid] = np.arange(self.nfluid, dtype=np.int32)
+
+    def coords(self):
+        c = self.cell_of_fluid.astype(np.int64)
+        return c % self.nx, (c // self.nx) % self.ny, c // (self.nx * self.ny)
+
+    def adjacency(self, q, layout=SOA, mode=ADJ_PULL):
+        adj = np.zeros(self.nfluid * q, np.int32)
+        rc = lib().orc_build_adjacency(q, layout, mode, self.nx, self.ny, self.nz,
+                                       _ptr(self.periodic), _ptr(self.fluid_of_cell),
+                                       _ptr(self.cell_of_fluid), self.nfluid, _ptr(adj))
+        if rc != 0:
+            raise ValueError("adjacency build failed")
+        return adj
+
+
+# --------------------------------------------------------------------------- states
+def equilibrium(q, rho, u):
+    """f^eq_s(rho, u) for arrays rho[n], u[n,3] -> f[q, n] (canonical SOA), with
+    the collision's association (src/kernels/KernelAA_SOA_Base.cpp:227-238, F = 0)."""
+    dirs, _, w = lattice(q)
+    rho = np.asarray(rho, np.float64)
+    u = np.asarray(u, np.float64)
+    vdotv = 3.0 * (u[:, 0] * u[:, 0] + u[:, 1] * u[:, 1] + u[:, 2] * u[:, 2])
+    out = np.empty((q, rho.size), np.float64)
+    for s in range(q):
+        c = dirs[s].astype(np.float64)
+        cdotv = 3.0 * (c[0] * u[:, 0] + c[1] * u[:, 1] + c[2] * u[:, 2])
+        out[s] = w[s] * rho * (1.0 + cdotv + 0.5 * cdotv * cdotv - 0.5 * vdotv)
+    return out
+
+
+def tgv_velocity(geom: DenseGeometry, u0=0.05, noise=1e-4, seed=1):
+    """Taylor-Green vortex u = u0 sin(kx)cos(ky)cos(kz), v = -u0 cos(kx)sin(ky)cos(kz),
+    w = 0 (k = 2 pi / n per axis), plus U(-noise, noise) per component drawn from
+    uniform01(seed, 3 * cell + component)."""
+    x, y, z = geom.coords()
+    kx, ky, kz = 2 * np.pi / geom.nx, 2 * np.pi / geom.ny, 2 * np.pi / geom.nz
+    u = np.zeros((geom.nfluid, 3), np.float64)
+    u[:, 0] = u0 * np.sin(kx * x) * np.cos(ky * y) * np.cos(kz * z)
+    u[:, 1] = -u0 * np.cos(kx * x) * np.sin(ky * y) * np.cos(kz * z)
+    if noise:
+        cell = geom.cell_of_fluid.astype(np.uint64)
+        for comp in range(3):
+            r = uniform01(seed, np.uint64(3) * cell + np.uint64(comp))
+            u[:, comp] += noise * (2.0 * r - 1.0)
+    return u
+
+
+def to_storage(f_canon, q, layout, opp=None, aa_even_ready=False):
+    """Canonical f[q, n] -> flat storage.  aa_even_ready puts f_s in slot opp(s)
+    (the AA state before an even step, src/propagation/PropagationAA.cpp:111)."""
+    f = np.asarray(f_canon)
+    if aa_even_ready:
+        if opp is None:
+            opp = lattice(q)[1]
+        g = np.empty_like(f)
+        g[opp] = f
+        f = g
+    return np.ascontiguousarray(f.reshape(q, -1) if layout == SOA else f.T).reshape(-1)
+
+
+def from_storage(flat, q, n, layout):
+    a = np.asarray(flat).reshape(q, n) if layout == SOA else np.asarray(flat).reshape(n, q).T
+    return np.ascontiguousarray(a)
+
+
+def aa_canonical(flat, q, n, layout, adj_pull, t):
+    """Canonical f[q,n] from an AA array after t completed steps: t even -> slot
+    opp(s) at the node; t odd -> through the AA adjacency
+    (src/propagation/PropagationAA.cpp:218-327)."""
+    flat = np.asarray(flat)
+    _, opp, _ = lattice(q)
+    if t % 2 == 0:
+        a = from_storage(flat, q, n, layout)
+        return a[opp]
+    out = np.empty((q, n), flat.dtype)
+    adjm = from_storage(adj_pull, q, n, layout)
+    for s in range(q):
+        out[s] = flat[adjm[s]]
+    return out
+
+
+# --------------------------------------------------------------------------- runners
+def _f(a):
+    return np.ascontiguousarray(a)
+
+
+def run_aa(q, layout, geom_n, f, adj, steps, omega, force=(1e-6, 0.0, 0.0), t0=0,
+           omega_node=None, gamma_node=None):
+    """Runs `steps` AA steps in place on storage array f (float64 or float32)."""
+    L = lib()
+    suf = "f64" if f.dtype == np.float64 else "f32"
+    F = np.asarray(force, np.float64)
+    on = None if omega_node is None else _f(np.asarray(omega_node, np.float64))
+    gn = None if gamma_node is None else _f(np.asarray(gamma_node, np.float64))
+    even = getattr(L, f"orc_aa_even_{suf}")
+    odd = getattr(L, f"orc_aa_odd_{suf}")
+    for t in range(t0, t0 + steps):
+        if t % 2 == 0:
+            even(q, layout, geom_n, _ptr(f), 0, geom_n, omega, _ptr(F), _ptr(on), _ptr(gn))
+        else:
+            odd(q, layout, geom_n, _ptr(f), _ptr(adj), 0, geom_n, omega, _ptr(F), _ptr(on), _ptr(gn))
+    return f
+
+
+def run_ab(q, layout, geom_n, f, adj, steps, omega, pull=False, force=(1e-6, 0.0, 0.0),
+           omega_node=None, gamma_node=None):
+    """Runs `steps` AB steps (push, or pull) starting from f; returns the latest state."""
+    L = lib()
+    suf = "f64" if f.dtype == np.float64 else "f32"
+    fn = getattr(L, f"orc_ab_{'pull' if pull else 'push'}_{suf}")
+    F = np.asarray(force, np.float64)
+    on = None if omega_node is None else _f(np.asarray(omega_node, np.float64))
+    gn = None if gamma_node is None else _f(np.asarray(gamma_node, np.float64))
+    src = f.copy()
+    tgt = f.copy()
+    for _ in range(steps):
+        fn(q, layout, geom_n, _ptr(src), _ptr(tgt), _ptr(adj), 0, geom_n, omega, _ptr(F),
+           _ptr(on), _ptr(gn))
+        src, tgt = tgt, src
+    return src
+
+
+def run_unroll19(kind, n, f, adj, steps, omega, t0=0):
+    """Unrolled Q19 SOA variants: kind in {"aa", "ab_push", "ab_pull"}."""
+    L = lib()
+    suf = "f64" if f.dtype == np.float64 else "f32"
+    if kind == "aa":
+        for t in range(t0, t0 + steps):
+            if t % 2 == 0:
+                getattr(L, f"orc_aa_even_unroll19_{suf}")(n, _ptr(f), 0, n, omega)
+            else:
+                getattr(L, f"orc_aa_odd_unroll19_{suf}")(n, _ptr(f), _ptr(adj), 0, n, omega)
+        return f
+    fn = getattr(L, f"orc_{kind}_unroll19_{suf}")
+    src, tgt = f.copy(), f.copy()
+    for _ in range(steps):
+        fn(n, _ptr(src), _ptr(tgt), _ptr(adj), 0, n, omega)
+        src, tgt = tgt, src
+    return src
+
+
+def adjoint(q, geom: DenseGeometry, f0_canon, nsteps, omega, force, tau_node=None,
+            gamma_node=None, objective=0, outlet_x=None):
+    """FP64 adjoint of the canonical push map.  Returns (J, grad_gamma, grad_tau, f_final)."""
+    L = lib()
+    n = geom.nfluid
+    adj = geom.adjacency(q, SOA, ADJ_PUSH)
+    f0 = _f(np.asarray(f0_canon, np.float64).reshape(-1))
+    F = np.asarray(force, np.float64)
+    tn = None if tau_node is None else _f(np.asarray(tau_node, np.float64))
+    gn = None if gamma_node is None else _f(np.asarray(gamma_node, np.float64))
+    gg = np.zeros(n)
+    gt = np.zeros(n)
+    ff = np.zeros(n * q)
+    J = np.zeros(1)
+    ox = geom.nx - 1 if outlet_x is None else outlet_x
+    rc = L.orc_adjoint(q, n, _ptr(adj), _ptr(f0), nsteps, omega, _ptr(F), _ptr(tn), _ptr(gn),
+                       objective, _ptr(geom.cell_of_fluid), geom.nx, ox, _ptr(ff), _ptr(J),
+                       _ptr(gg), _ptr(gt))
+    if rc != 0:
+        raise RuntimeError(f"orc_adjoint failed: {rc}")
+    return float(J[0]), gg, gt, ff.reshape(q, n)
+
+
+def objective(q, geom: DenseGeometry, f_canon, kind=0, outlet_x=None):
+    L = lib()
+    ox = geom.nx - 1 if outlet_x is None else outlet_x
+    f = _f(np.asarray(f_canon, np.float64).reshape(-1))
+    return float(L.orc_objective(q, kind, geom.nfluid, _ptr(f), _ptr(geom.cell_of_fluid),
+                                 geom.nx, ox))
+
+
+# --------------------------------------------------------------------------- reference
+REF_VARIANTS = {"aa_soa_base": 0, "aa_soa_unroll": 1, "aa_aos_base": 2, "ab_soa_base": 3,
+                "ab_soa_pull": 4, "ab_soa_unroll": 5, "ab_soa_unroll_pull": 6, "ab_aos_base": 7}
+OMEGA_TAGS = {1.0: "w1", 1 / 0.6: "w0p6", 1 / 0.8: "w0p8", 1 / 0.51: "w0p51"}
+
+
+def ref_available(q=19, prec="f64", omega=1.0) -> bool:
+    tag = OMEGA_TAGS.get(omega)
+    return tag is not None and os.path.exists(os.path.join(REF_DIR, f"libref_q{q}_{prec}_{tag}.so"))
+
+
+_ref_libs = {}
+
+
+def ref_lib(q=19, prec="f64", omega=1.0):
+    tag = OMEGA_TAGS[omega]
+    key = (q, prec, tag)
+    if key not in _ref_libs:
+        L = ctypes.CDLL(os.path.join(REF_DIR, f"libref_q{q}_{prec}_{tag}.so"))
+        L.ref_run.argtypes = [_I, _I, _P, _P, _P, _I, _I]
+        L.ref_run.restype = _D
+        L.ref_lattice.argtypes = [_P, _P, _P]
+        L.ref_omega.restype = _D
+        _ref_libs[key] = L
+    return _ref_libs[key]
+
+
+def ref_run(variant, q, prec, omega, n, f, adj, steps, t0=0):
+    """Runs the reference's own kernel `variant` for `steps` steps; f updated in place.
+    Returns the wall time of the step loop (seconds)."""
+    L = ref_lib(q, prec, omega)
+    assert abs(L.ref_omega() - omega) == 0.0
+    f2 = f.copy()
+    adj = np.ascontiguousarray(adj, np.int32)
+    dt = L.ref_run(REF_VARIANTS[variant], n, _ptr(adj), _ptr(f), _ptr(f2), steps, t0)
+    if dt < 0:
+        raise RuntimeError("ref_run failed")
+    return dt

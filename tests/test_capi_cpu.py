@@ -1,0 +1,79 @@
+"""The C-ABI library without a GPU: it loads, exports every entry point that
+include/lbm_capi.h declares, and rejects invalid arguments with status codes
+before any device call (no compute runs here)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from lbmminiapp_amd import capi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "lbm_capi.h")
+
+
+def header_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(lbm_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(capi.LIB_PATH):
+        from lbmminiapp_amd import build
+        build.build()
+    return capi.load()
+
+
+def test_header_declares_expected_entry_points():
+    names = header_functions()
+    assert set(capi.EXPORTED) == set(names), set(capi.EXPORTED) ^ set(names)
+
+
+def test_library_exports_every_declared_symbol(lib):
+    missing = [n for n in header_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_version_and_error_string(lib):
+    assert b"gfx950" in lib.lbm_version()
+    assert isinstance(lib.lbm_last_error(), bytes)
+
+
+def test_dense_create_rejects_bad_arguments(lib):
+    h = ctypes.c_void_p()
+    base = dict(nx=8, ny=8, nz=8, q=19, pattern=1, pull=0, layout=0, storage=0,
+                periodic=(ctypes.c_int32 * 3)(1, 1, 1), omega=1.0,
+                force=(ctypes.c_double * 3)(0, 0, 0), solid=None, tau=None, gamma=None,
+                f16_scale=0.0)
+    for bad in (dict(q=20), dict(nx=0), dict(pattern=7), dict(layout=3), dict(storage=9),
+                dict(omega=2.5), dict(omega=float("nan")), dict(ny=70000)):
+        cfg = capi._DenseConfig(**{**base, **bad})
+        assert lib.lbm_dense_create(ctypes.byref(cfg), ctypes.byref(h)) == capi.ERR_ARG, bad
+        assert lib.lbm_last_error()
+    assert lib.lbm_dense_create(None, ctypes.byref(h)) == capi.ERR_ARG
+
+
+def test_sparse_create_rejects_bad_adjacency(lib):
+    import numpy as np
+    adj = np.zeros(19 * 4, np.int32)
+    adj[5] = 19 * 4  # out of range
+    cfg = capi._SparseConfig(4, 19, 0, capi.STORE_F64, 1, 1.0, (ctypes.c_double * 3)(1e-6, 0, 0),
+                             adj.ctypes.data_as(ctypes.c_void_p))
+    h = ctypes.c_void_p()
+    assert lib.lbm_sparse_create(ctypes.byref(cfg), ctypes.byref(h)) == capi.ERR_ARG
+    assert b"adjacency" in lib.lbm_last_error()
+    cfg.q = 21
+    assert lib.lbm_sparse_create(ctypes.byref(cfg), ctypes.byref(h)) == capi.ERR_ARG
+    cfg.q = 19
+    cfg.storage = capi.STORE_F16S
+    assert lib.lbm_sparse_create(ctypes.byref(cfg), ctypes.byref(h)) == capi.ERR_ARG
+
+
+def test_null_handles_are_rejected(lib):
+    assert lib.lbm_dense_step(None, 1) == capi.ERR_ARG
+    assert lib.lbm_dense_mass(None, None) == capi.ERR_ARG
+    assert lib.lbm_sparse_aa_even(None, None, 0, 1) == capi.ERR_ARG
+    assert lib.lbm_copy_bandwidth(15, 1, None) == capi.ERR_ARG
