@@ -45,17 +45,20 @@ def parse():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--q", type=int, default=19)
-    ap.add_argument("--legs", default="f64,f16s,f32s")
+    ap.add_argument("--legs", default="f64,f16s,f32s,f64_force")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     return ap.parse_args()
 
 
-def gpu_leg(storage, n, q, steps, warmup):
+def gpu_leg(leg, n, q, steps, warmup):
+    """leg: f64 | f32s | f16s (Taylor-Green, F = 0) or f64_force (the reference's
+    body force F = (1e-6, 0, 0) switched on, same initial state)."""
     from lbmminiapp_amd.capi import DenseLattice
 
-    lat = DenseLattice(n, n, n, q=q, pattern="aa", storage=storage, omega=1.0 / TAU,
-                       force=(0.0, 0.0, 0.0))
+    storage = leg.split("_")[0]
+    force = (1e-6, 0.0, 0.0) if leg.endswith("_force") else (0.0, 0.0, 0.0)
+    lat = DenseLattice(n, n, n, q=q, pattern="aa", storage=storage, omega=1.0 / TAU, force=force)
     try:
         lat.init_tgv(0.05, 1e-4, 1)
         lat.step(warmup)
@@ -83,6 +86,7 @@ def gpu_leg(storage, n, q, steps, warmup):
     achieved = bpl * nf / launch_s / 1e9
     return {
         "storage": storage,
+        "force": force[0],
         "mlups": nf * steps / wall / 1e6,
         "ms_per_step": wall * 1e3 / steps,
         "event_ms_per_launch": ev_ms / steps,

@@ -171,6 +171,25 @@ lbm_status lbm_dense_moments(lbm_dense* h, double* rho, double* u);
 lbm_status lbm_dense_mass(lbm_dense* h, double* mass);
 lbm_status lbm_dense_synchronize(lbm_dense* h);
 
+/* ---------------------------------------------------------------- geometry and fields
+ * Host-side generators (no device work).  Replace Geometry::setupGrid
+ * (src/geometries/Geometry.cpp:100-151), whose only case is the cylinder.
+ * Masks: solid[x + nx * (y + ny * z)] != 0 marks a solid cell. */
+/* Reference cylinder for simulationSize s (src/geometries/Geometry.cpp:60-70,113-131):
+ * dims = {int(84 s), 2 int(8 s) + 3, 2 int(8 s) + 3}; solid may be NULL to query dims. */
+lbm_status lbm_geometry_cylinder(double simulation_size, int32_t* dims, uint8_t* solid);
+/* Solid layers at y = 0 and y = ny - 1. */
+lbm_status lbm_geometry_channel(int32_t nx, int32_t ny, int32_t nz, uint8_t* solid);
+/* Seeded random spheres (radius in [rmin, rmax], wrapping periodic faces) added to
+ * `solid` until the solid fraction reaches `fraction`; *count = spheres drawn. */
+lbm_status lbm_geometry_add_spheres(int32_t nx, int32_t ny, int32_t nz, const int32_t* periodic,
+                                    double rmin, double rmax, double fraction, uint64_t seed,
+                                    uint8_t* solid, int64_t* count);
+/* Seeded smooth field in [0, 1]: white noise, periodic Gaussian filter (sigma,
+ * truncated at 4 sigma), min-max rescaled. */
+lbm_status lbm_field_smooth_random(int32_t nx, int32_t ny, int32_t nz, double sigma,
+                                   uint64_t seed, double* out);
+
 /* ---------------------------------------------------------------- measurement helper */
 /* Device-to-device copy kernel of `bytes` (multiple of 16) between two buffers
  * allocated by the library; used to calibrate achieved-bandwidth and

@@ -31,7 +31,8 @@ EXPORTED = [
     "lbm_dense_create", "lbm_dense_destroy", "lbm_dense_get_info", "lbm_dense_set_state",
     "lbm_dense_get_state", "lbm_dense_init_equilibrium", "lbm_dense_init_tgv", "lbm_dense_step",
     "lbm_dense_run_timed", "lbm_dense_moments", "lbm_dense_mass", "lbm_dense_synchronize",
-    "lbm_copy_bandwidth",
+    "lbm_geometry_cylinder", "lbm_geometry_channel", "lbm_geometry_add_spheres",
+    "lbm_field_smooth_random", "lbm_copy_bandwidth",
 ]
 
 
@@ -93,6 +94,9 @@ def load(path: str = LIB_PATH):
         "lbm_dense_step": [P, I32], "lbm_dense_run_timed": [P, I32, P],
         "lbm_dense_moments": [P, P, P], "lbm_dense_mass": [P, P], "lbm_dense_synchronize": [P],
         "lbm_copy_bandwidth": [ctypes.c_size_t, I32, P],
+        "lbm_geometry_cylinder": [D, P, P], "lbm_geometry_channel": [I32, I32, I32, P],
+        "lbm_geometry_add_spheres": [I32, I32, I32, P, D, D, D, ctypes.c_uint64, P, P],
+        "lbm_field_smooth_random": [I32, I32, I32, D, ctypes.c_uint64, P],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -123,6 +127,38 @@ def copy_bandwidth(nbytes: int, reps: int = 10) -> float:
     ms = ctypes.c_double(0)
     check(load().lbm_copy_bandwidth(nbytes, reps, ctypes.byref(ms)))
     return 2.0 * nbytes * reps / (ms.value * 1e-3) / 1e9
+
+
+# --------------------------------------------------------------------------- geometry
+def cylinder(simulation_size: float):
+    """Reference cylinder mask; returns (dims (nx, ny, nz), solid uint8[ncell])."""
+    dims = np.zeros(3, np.int32)
+    check(load().lbm_geometry_cylinder(float(simulation_size), _p(dims), None))
+    solid = np.zeros(int(np.prod(dims.astype(np.int64))), np.uint8)
+    check(load().lbm_geometry_cylinder(float(simulation_size), _p(dims), _p(solid)))
+    return tuple(int(d) for d in dims), solid
+
+
+def channel(nx, ny, nz):
+    solid = np.zeros(nx * ny * nz, np.uint8)
+    check(load().lbm_geometry_channel(nx, ny, nz, _p(solid)))
+    return solid
+
+
+def add_spheres(solid, nx, ny, nz, rmin, rmax, fraction, seed, periodic=(1, 1, 1)):
+    """Adds seeded random spheres to `solid` in place; returns the number drawn."""
+    per = np.asarray(periodic, np.int32)
+    count = ctypes.c_int64(0)
+    check(load().lbm_geometry_add_spheres(nx, ny, nz, _p(per), float(rmin), float(rmax),
+                                          float(fraction), int(seed), _p(solid),
+                                          ctypes.byref(count)))
+    return count.value
+
+
+def smooth_random_field(nx, ny, nz, sigma, seed):
+    out = np.empty(nx * ny * nz, np.float64)
+    check(load().lbm_field_smooth_random(nx, ny, nz, float(sigma), int(seed), _p(out)))
+    return out
 
 
 # --------------------------------------------------------------------------- device buffers

@@ -69,13 +69,24 @@ __host__ __device__ inline void moments_acc(const T* f, T& rho, T& v0, T& v1, T&
   if constexpr (S + 1 < Q) moments_acc<Q, T, S + 1>(f, rho, v0, v1, v2);
 }
 
-template <int Q, typename T, int S>
+template <int Q, typename T, int S, bool FORCE>
 __host__ __device__ inline void relax_ref(const T* f, T* out, T rho, T v0, T v1, T v2, T vdotv,
                                           T fdotv, const T* F, double omega) {
 #pragma clang fp contract(off)
   using D = Dir<Q, S>;
   const T w = T(Lattice<Q>::w(S));
-  if constexpr (D::cx == 0 && D::cy == 0 && D::cz == 0) {
+  if constexpr (!FORCE) {
+    // F = 0: every Guo term is an exact +0 (cdotf = fdotv = 0), so dropping
+    // them leaves the reference result unchanged bit for bit.
+    if constexpr (D::cx == 0 && D::cy == 0 && D::cz == 0) {
+      T eq = w * rho * (1.0 - 0.5 * vdotv);
+      out[S] = (1.0 - omega) * f[S] + omega * eq;
+    } else {
+      T cdotv = 3.0 * csum<T, D::cx, D::cy, D::cz>(v0, v1, v2);
+      T eq = w * rho * (1.0 + cdotv + 0.5 * cdotv * cdotv - 0.5 * vdotv);
+      out[S] = (1.0 - omega) * f[S] + omega * eq;
+    }
+  } else if constexpr (D::cx == 0 && D::cy == 0 && D::cz == 0) {
     // rest direction: cdotv = cdotf = 0, (1 + 0) + 0 - x == 1 - x exactly
     T eq = w * rho * (1.0 - 0.5 * vdotv);
     T fdist = w * (T(0) - fdotv);
@@ -87,20 +98,24 @@ __host__ __device__ inline void relax_ref(const T* f, T* out, T rho, T v0, T v1,
     T fdist = w * (cdotf - fdotv + cdotv * cdotf);
     out[S] = (1.0 - omega) * f[S] + omega * eq + (1.0 - 0.5 * omega) * fdist;
   }
-  if constexpr (S + 1 < Q) relax_ref<Q, T, S + 1>(f, out, rho, v0, v1, v2, vdotv, fdotv, F, omega);
+  if constexpr (S + 1 < Q)
+    relax_ref<Q, T, S + 1, FORCE>(f, out, rho, v0, v1, v2, vdotv, fdotv, F, omega);
 }
 
 // In-place or out-of-place: out may alias f (each f[S] is read before out[S] is written).
-template <int Q, typename T, bool BRINKMAN>
+// FORCE = false requires F = (0, 0, 0) and then equals FORCE = true bit for bit.
+template <int Q, typename T, bool BRINKMAN, bool FORCE = true>
 __host__ __device__ inline void collide_ref(const T* f, T* out, double omega, const T* F,
                                             double gamma) {
 #pragma clang fp contract(off)
   T rho = 0.0;
   T v0 = 0.0, v1 = 0.0, v2 = 0.0;
   moments_acc<Q, T, 0>(f, rho, v0, v1, v2);
-  v0 += 0.5 * F[0];
-  v1 += 0.5 * F[1];
-  v2 += 0.5 * F[2];
+  if constexpr (FORCE) {
+    v0 += 0.5 * F[0];
+    v1 += 0.5 * F[1];
+    v2 += 0.5 * F[2];
+  }
   T invRho = 1.0 / rho;
   v0 *= invRho;
   v1 *= invRho;
@@ -112,8 +127,8 @@ __host__ __device__ inline void collide_ref(const T* f, T* out, double omega, co
     v2 *= scale;
   }
   T vdotv = 3.0 * (v0 * v0 + v1 * v1 + v2 * v2);
-  T fdotv = 3.0 * (F[0] * v0 + F[1] * v1 + F[2] * v2);
-  relax_ref<Q, T, 0>(f, out, rho, v0, v1, v2, vdotv, fdotv, F, omega);
+  T fdotv = FORCE ? T(3.0 * (F[0] * v0 + F[1] * v1 + F[2] * v2)) : T(0);
+  relax_ref<Q, T, 0, FORCE>(f, out, rho, v0, v1, v2, vdotv, fdotv, F, omega);
 }
 
 // ---------------------------------------------------------------- shifted storage
@@ -132,7 +147,7 @@ __host__ __device__ inline void moments_shift(const float* d, double& drho, doub
   if constexpr (S + 1 < Q) moments_shift<Q, S + 1>(d, drho, m0, m1, m2);
 }
 
-template <int Q, int S>
+template <int Q, int S, bool FORCE>
 __host__ __device__ inline void relax_shift(float* d, float drho, float rho, float u0, float u1,
                                             float u2, float uu, float fu, float F0, float F1,
                                             float F2, float om, float omf) {
@@ -142,26 +157,30 @@ __host__ __device__ inline void relax_shift(float* d, float drho, float rho, flo
   float cu = 0.0f, cf = 0.0f;
   if constexpr (!(D::cx == 0 && D::cy == 0 && D::cz == 0)) {
     cu = 3.0f * csum<float, D::cx, D::cy, D::cz>(u0, u1, u2);
-    cf = 3.0f * csum<float, D::cx, D::cy, D::cz>(F0, F1, F2);
+    if constexpr (FORCE) cf = 3.0f * csum<float, D::cx, D::cy, D::cz>(F0, F1, F2);
   }
   const float eqdev = drho + rho * (cu + 0.5f * cu * cu - 0.5f * uu);
-  const float fd = cf - fu + cu * cf;
-  d[S] = (1.0f - om) * d[S] + w * (om * eqdev + omf * fd);
+  if constexpr (FORCE) {
+    const float fd = cf - fu + cu * cf;
+    d[S] = (1.0f - om) * d[S] + w * (om * eqdev + omf * fd);
+  } else {
+    d[S] = (1.0f - om) * d[S] + w * (om * eqdev);
+  }
   if constexpr (S + 1 < Q)
-    relax_shift<Q, S + 1>(d, drho, rho, u0, u1, u2, uu, fu, F0, F1, F2, om, omf);
+    relax_shift<Q, S + 1, FORCE>(d, drho, rho, u0, u1, u2, uu, fu, F0, F1, F2, om, omf);
 }
 
-// d[Q]: deviations f_s - w_s in FP32, updated in place.
-template <int Q, bool BRINKMAN>
+// d[Q]: deviations f_s - w_s in FP32, updated in place.  FORCE = false requires F = 0.
+template <int Q, bool BRINKMAN, bool FORCE = true>
 __host__ __device__ inline void collide_shifted(float* d, double omega, const double* F,
                                                 double gamma) {
   double drho = 0.0, m0 = 0.0, m1 = 0.0, m2 = 0.0;
   moments_shift<Q, 0>(d, drho, m0, m1, m2);
   const double rho = 1.0 + drho;
   const double inv = 1.0 / rho;
-  double u0 = (m0 + 0.5 * F[0]) * inv;
-  double u1 = (m1 + 0.5 * F[1]) * inv;
-  double u2 = (m2 + 0.5 * F[2]) * inv;
+  double u0 = (FORCE ? m0 + 0.5 * F[0] : m0) * inv;
+  double u1 = (FORCE ? m1 + 0.5 * F[1] : m1) * inv;
+  double u2 = (FORCE ? m2 + 0.5 * F[2] : m2) * inv;
   if constexpr (BRINKMAN) {
     u0 *= (1.0 - gamma);
     u1 *= (1.0 - gamma);
@@ -173,7 +192,8 @@ __host__ __device__ inline void collide_shifted(float* d, double omega, const do
   const float fu = 3.0f * (F0 * fu0 + F1 * fu1 + F2 * fu2);
   const float om = (float)omega;
   const float omf = 1.0f - 0.5f * om;
-  relax_shift<Q, 0>(d, (float)drho, (float)rho, fu0, fu1, fu2, uu, fu, F0, F1, F2, om, omf);
+  relax_shift<Q, 0, FORCE>(d, (float)drho, (float)rho, fu0, fu1, fu2, uu, fu, F0, F1, F2, om,
+                           omf);
 }
 
 }  // namespace lbm

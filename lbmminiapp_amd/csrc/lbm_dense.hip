@@ -11,6 +11,7 @@
 //   AB pull         the current array holds post-collision values; f_s(x) is gathered
 //                   like the AA odd case (src/kernels/KernelAB_SOA_Pull.cpp:139-140)
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "lbm_dense_kernels.hpp"
@@ -300,6 +301,7 @@ struct lbm_dense {
 
 namespace {
 
+// Utility kernels: 256-wide x segments.
 dim3 block_for(int nx) {
   int bx = nx >= 256 ? 256 : ((nx + 63) / 64) * 64;
   return dim3(bx);
@@ -310,6 +312,22 @@ dim3 grid_for(const lbm_dense* h) {
   return dim3((h->nx + b.x - 1) / b.x, h->ny, h->nz);
 }
 
+// Time-step kernels: a whole x-row per workgroup when it fits (kDenseMaxBlock),
+// else 256-wide segments.  LBM_DENSE_BLOCK_X overrides the width (tuning).
+dim3 step_block_for(int nx, int storage) {
+  static const int forced = [] {
+    const char* e = std::getenv("LBM_DENSE_BLOCK_X");
+    return e ? std::atoi(e) : 0;
+  }();
+  int bx;
+  if (forced > 0) bx = forced;
+  else if (nx <= kDenseMaxBlock && storage != LBM_STORE_F32S) bx = ((nx + 63) / 64) * 64;
+  else bx = 256;
+  if (bx > kDenseMaxBlock) bx = kDenseMaxBlock;
+  if (bx < 64) bx = 64;
+  return dim3(bx);
+}
+
 // state convention of the current array
 int canon_mode(const lbm_dense* h) {
   if (h->pattern == LBM_PATTERN_AA) return (h->t % 2 == 0) ? C_OPP : C_GATHER;
@@ -318,25 +336,51 @@ int canon_mode(const lbm_dense* h) {
 
 DenseLaunchFn pick_step(const lbm_dense* h, int kind) {
   const bool mask = h->links != nullptr;
-  return h->q == 19 ? pick_dense19(h->storage, h->layout, kind, mask)
-                    : pick_dense27(h->storage, h->layout, kind, mask);
+  const bool force = h->args.F0 != 0.0 || h->args.F1 != 0.0 || h->args.F2 != 0.0;
+  return h->q == 19 ? pick_dense19(h->storage, h->layout, kind, mask, force)
+                    : pick_dense27(h->storage, h->layout, kind, mask, force);
+}
+
+// The LDS-staged kernels apply to fully periodic SOA boxes whose rows fit one
+// workgroup (nx a multiple of 64, nx <= kDenseMaxBlock).  LBM_DENSE_STAGED=0
+// forces the unstaged kernels (tuning and cross-checks).
+DenseLaunchFn pick_staged(const lbm_dense* h, int kind) {
+  static const bool enabled = [] {
+    const char* e = std::getenv("LBM_DENSE_STAGED");
+    return !(e && e[0] == '0');
+  }();
+  if (!enabled || h->links || h->layout != LBM_LAYOUT_SOA || h->nx % 64 != 0 ||
+      h->nx > kDenseMaxBlock)
+    return nullptr;
+  const bool force = h->args.F0 != 0.0 || h->args.F1 != 0.0 || h->args.F2 != 0.0;
+  return h->q == 19 ? pick_staged19(h->storage, kind, force, h->nx)
+                    : pick_staged27(h->storage, kind, force, h->nx);
+}
+
+lbm_status launch_kind(lbm_dense* h, int kind, void* f0, void* f1) {
+  if (DenseLaunchFn st = pick_staged(h, kind)) {
+    st(dim3(1, h->ny, h->nz), dim3(h->nx), h->stream, f0, f1, h->args);
+  } else {
+    DenseLaunchFn fn = pick_step(h, kind);
+    LBM_REQUIRE(fn, LBM_ERR_UNSUPPORTED, "no kernel for this configuration");
+    const dim3 b = step_block_for(h->nx, h->storage);
+    const dim3 g((h->nx + b.x - 1) / b.x, h->ny, h->nz);
+    fn(g, b, h->stream, f0, f1, h->args);
+  }
+  LBM_LAUNCH_CHECK();
+  return LBM_OK;
 }
 
 lbm_status enqueue_steps(lbm_dense* h, int nsteps) {
-  const dim3 g = grid_for(h), b = block_for(h->nx);
   for (int i = 0; i < nsteps; i++) {
+    lbm_status st;
     if (h->pattern == LBM_PATTERN_AA) {
-      const int kind = (h->t % 2 == 0) ? K_AA_EVEN : K_AA_ODD;
-      DenseLaunchFn fn = pick_step(h, kind);
-      LBM_REQUIRE(fn, LBM_ERR_UNSUPPORTED, "no kernel for this configuration");
-      fn(g, b, h->stream, h->f[0], h->f[0], h->args);
+      st = launch_kind(h, (h->t % 2 == 0) ? K_AA_EVEN : K_AA_ODD, h->f[0], h->f[0]);
     } else {
-      DenseLaunchFn fn = pick_step(h, h->pull ? K_AB_PULL : K_AB_PUSH);
-      LBM_REQUIRE(fn, LBM_ERR_UNSUPPORTED, "no kernel for this configuration");
-      fn(g, b, h->stream, h->f[h->cur], h->f[1 - h->cur], h->args);
+      st = launch_kind(h, h->pull ? K_AB_PULL : K_AB_PUSH, h->f[h->cur], h->f[1 - h->cur]);
       h->cur = 1 - h->cur;
     }
-    LBM_LAUNCH_CHECK();
+    if (st != LBM_OK) return st;
     h->t++;
   }
   return LBM_OK;

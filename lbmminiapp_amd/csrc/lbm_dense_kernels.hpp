@@ -80,6 +80,23 @@ __device__ __forceinline__ void st(typename Storage<STORE>::T* p, int64_t idx,
   else p[idx] = v;
 }
 
+// Streaming (non-temporal) variants for the time-step kernels: every population
+// is touched once per step, so there is nothing to keep in the caches.
+template <int STORE>
+__device__ __forceinline__ typename Storage<STORE>::C ld_nt(const typename Storage<STORE>::T* p,
+                                                            int64_t idx, float inv_scale) {
+  const typename Storage<STORE>::T v = __builtin_nontemporal_load(p + idx);
+  if constexpr (STORE == LBM_STORE_F16S) return (float)v * inv_scale;
+  else return v;
+}
+
+template <int STORE>
+__device__ __forceinline__ void st_nt(typename Storage<STORE>::T* p, int64_t idx,
+                                      typename Storage<STORE>::C v, float scale) {
+  if constexpr (STORE == LBM_STORE_F16S) __builtin_nontemporal_store((_Float16)(v * scale), p + idx);
+  else __builtin_nontemporal_store(v, p + idx);
+}
+
 template <int Q, int LAYOUT>
 __device__ __forceinline__ int64_t slot(int64_t ncell, int s, int64_t cell) {
   return LAYOUT == LBM_LAYOUT_SOA ? ncell * s + cell : cell * Q + s;
@@ -106,8 +123,13 @@ struct Offsets {
   }
 };
 
-template <int Q, int STORE, int LAYOUT, int KIND, bool MASK>
-__global__ void __launch_bounds__(256)
+// Blocks are one x-row segment of up to kDenseMaxBlock cells (a full row when
+// nx <= kDenseMaxBlock, so that every cache line shared by x-shifted accesses is
+// used by one workgroup).
+constexpr int kDenseMaxBlock = 512;
+
+template <int Q, int STORE, int LAYOUT, int KIND, bool MASK, bool FORCE>
+__global__ void __launch_bounds__(kDenseMaxBlock)
     k_dense(const typename Storage<STORE>::T* f0, typename Storage<STORE>::T* f1, DenseArgs a) {
   using C = typename Storage<STORE>::C;
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -128,25 +150,31 @@ __global__ void __launch_bounds__(256)
     const int o = Lattice<Q>::opp(s);
     const int cx = Lattice<Q>::c(s, 0), cy = Lattice<Q>::c(s, 1), cz = Lattice<Q>::c(s, 2);
     int64_t idx;
+    // Row-aligned accesses stream past the caches; x-shifted ones (c_x != 0 in
+    // the gathering kinds) share their boundary cache line with the
+    // neighbouring wavefront and must stay cacheable.
+    bool aligned = true;
     if constexpr (KIND == K_AA_EVEN) {
       idx = slot<Q, LAYOUT>(n, o, cell);
     } else if constexpr (KIND == K_AB_PUSH) {
       idx = slot<Q, LAYOUT>(n, s, cell);
     } else {
+      aligned = cx == 0;
       if (MASK && ((lk >> o) & 1u)) idx = slot<Q, LAYOUT>(n, o, cell);
       else idx = slot<Q, LAYOUT>(n, s, cell + off.at(-cx, -cy, -cz));
     }
-    d[s] = ld<STORE>(f0, idx, a.inv_scale);
+    if (aligned && LAYOUT == LBM_LAYOUT_SOA) d[s] = ld_nt<STORE>(f0, idx, a.inv_scale);
+    else d[s] = ld<STORE>(f0, idx, a.inv_scale);
   }
 
   const double om = a.omega_field ? a.omega_field[cell] : a.omega;
   const double F[3] = {a.F0, a.F1, a.F2};
   if constexpr (STORE == LBM_STORE_F64) {
-    if (a.gamma_field) collide_ref<Q, double, true>(d, d, om, F, a.gamma_field[cell]);
-    else collide_ref<Q, double, false>(d, d, om, F, 0.0);
+    if (a.gamma_field) collide_ref<Q, double, true, FORCE>(d, d, om, F, a.gamma_field[cell]);
+    else collide_ref<Q, double, false, FORCE>(d, d, om, F, 0.0);
   } else {
-    if (a.gamma_field) collide_shifted<Q, true>(d, om, F, a.gamma_field[cell]);
-    else collide_shifted<Q, false>(d, om, F, 0.0);
+    if (a.gamma_field) collide_shifted<Q, true, FORCE>(d, om, F, a.gamma_field[cell]);
+    else collide_shifted<Q, false, FORCE>(d, om, F, 0.0);
   }
 
 #pragma unroll
@@ -154,69 +182,234 @@ __global__ void __launch_bounds__(256)
     const int o = Lattice<Q>::opp(s);
     const int cx = Lattice<Q>::c(s, 0), cy = Lattice<Q>::c(s, 1), cz = Lattice<Q>::c(s, 2);
     int64_t idx;
+    bool aligned = true;
     if constexpr (KIND == K_AA_EVEN || KIND == K_AB_PULL) {
       idx = slot<Q, LAYOUT>(n, s, cell);
     } else {
+      aligned = cx == 0;
       const bool bb = MASK && ((lk >> s) & 1u);
       if constexpr (KIND == K_AA_ODD)
         idx = bb ? slot<Q, LAYOUT>(n, s, cell) : slot<Q, LAYOUT>(n, o, cell + off.at(cx, cy, cz));
       else
         idx = bb ? slot<Q, LAYOUT>(n, o, cell) : slot<Q, LAYOUT>(n, s, cell + off.at(cx, cy, cz));
     }
-    st<STORE>(f1, idx, d[s], a.scale);
+    if (aligned && LAYOUT == LBM_LAYOUT_SOA) st_nt<STORE>(f1, idx, d[s], a.scale);
+    else st<STORE>(f1, idx, d[s], a.scale);
   }
+}
+
+// number of directions with c_x != 0
+template <int Q>
+__host__ __device__ constexpr int n_xshift() {
+  int k = 0;
+  for (int s = 0; s < Q; s++) k += Lattice<Q>::c(s, 0) != 0;
+  return k;
+}
+
+// LDS-staged variant of the gathering / scattering kinds (AA_ODD, AB_PUSH,
+// AB_PULL) for fully periodic SOA boxes: one workgroup is one whole x-row
+// (blockDim.x == nx, nx a multiple of 64).  Every global access is the aligned
+// element x of some row, so each wavefront moves whole cache lines exactly
+// once (streamed past the caches).  The x-shift of the directions with
+// c_x != 0, including the periodic wrap at the row ends, is applied in LDS:
+//   gather : thread x stores element x of row (y-cy, z-cz) in LDS, node x then
+//            reads LDS[(x - cx) mod nx]
+//   scatter: node x writes its result to LDS[(x + cx) mod nx], thread x then
+//            stores LDS[x] as element x of row (y+cy, z+cz)
+// Same reads, writes and arithmetic as k_dense, so the results are identical.
+template <int Q, int STORE, int KIND, bool FORCE>
+__global__ void __launch_bounds__(kDenseMaxBlock)
+    k_dense_staged(const typename Storage<STORE>::T* f0, typename Storage<STORE>::T* f1,
+                   DenseArgs a) {
+  using C = typename Storage<STORE>::C;
+  constexpr int NSH = n_xshift<Q>();
+  extern __shared__ unsigned char lbm_smem[];
+  C* lds = reinterpret_cast<C*>(lbm_smem);  // [NSH][nx]
+  const int nx = a.nx;
+  const int x = threadIdx.x;
+  const int y = blockIdx.y, z = blockIdx.z;
+  const int64_t cell = x + (int64_t)nx * (y + (int64_t)a.ny * z);
+  const Offsets off(x, y, z, a.nx, a.ny, a.nz);
+  const int64_t n = a.ncell;
+  constexpr bool GATHER = KIND == K_AA_ODD || KIND == K_AB_PULL;
+  constexpr bool SCATTER = KIND == K_AA_ODD || KIND == K_AB_PUSH;
+
+  C d[Q];
+  {
+    int k = 0;
+#pragma unroll
+    for (int s = 0; s < Q; s++) {
+      const int cx = Lattice<Q>::c(s, 0), cy = Lattice<Q>::c(s, 1), cz = Lattice<Q>::c(s, 2);
+      if constexpr (GATHER) {
+        // element x of slot s in row (y - cy, z - cz)
+        const C v = ld_nt<STORE>(f0, slot<Q, LBM_LAYOUT_SOA>(n, s, cell + off.at(0, -cy, -cz)),
+                                 a.inv_scale);
+        if (cx == 0) d[s] = v;
+        else lds[(k++) * nx + x] = v;
+      } else {
+        d[s] = ld_nt<STORE>(f0, slot<Q, LBM_LAYOUT_SOA>(n, s, cell), a.inv_scale);
+      }
+    }
+  }
+  if constexpr (GATHER) {
+    __syncthreads();
+    int k = 0;
+#pragma unroll
+    for (int s = 0; s < Q; s++) {
+      const int cx = Lattice<Q>::c(s, 0);
+      if (cx != 0) {
+        int xs = x - cx;
+        xs = xs < 0 ? xs + nx : (xs >= nx ? xs - nx : xs);
+        d[s] = lds[(k++) * nx + xs];
+      }
+    }
+  }
+
+  const double om = a.omega_field ? a.omega_field[cell] : a.omega;
+  const double F[3] = {a.F0, a.F1, a.F2};
+  if constexpr (STORE == LBM_STORE_F64) {
+    if (a.gamma_field) collide_ref<Q, double, true, FORCE>(d, d, om, F, a.gamma_field[cell]);
+    else collide_ref<Q, double, false, FORCE>(d, d, om, F, 0.0);
+  } else {
+    if (a.gamma_field) collide_shifted<Q, true, FORCE>(d, om, F, a.gamma_field[cell]);
+    else collide_shifted<Q, false, FORCE>(d, om, F, 0.0);
+  }
+
+  if constexpr (SCATTER) {
+    if constexpr (GATHER) __syncthreads();  // every LDS read of the gather is done
+    int k = 0;
+#pragma unroll
+    for (int s = 0; s < Q; s++) {
+      const int o = Lattice<Q>::opp(s);
+      const int cx = Lattice<Q>::c(s, 0), cy = Lattice<Q>::c(s, 1), cz = Lattice<Q>::c(s, 2);
+      const int tgt = KIND == K_AA_ODD ? o : s;
+      if (cx == 0) {
+        st_nt<STORE>(f1, slot<Q, LBM_LAYOUT_SOA>(n, tgt, cell + off.at(0, cy, cz)), d[s], a.scale);
+      } else {
+        int xt = x + cx;
+        xt = xt < 0 ? xt + nx : (xt >= nx ? xt - nx : xt);
+        lds[(k++) * nx + xt] = d[s];
+      }
+    }
+    __syncthreads();
+    k = 0;
+#pragma unroll
+    for (int s = 0; s < Q; s++) {
+      const int o = Lattice<Q>::opp(s);
+      const int cx = Lattice<Q>::c(s, 0), cy = Lattice<Q>::c(s, 1), cz = Lattice<Q>::c(s, 2);
+      const int tgt = KIND == K_AA_ODD ? o : s;
+      if (cx != 0)
+        st_nt<STORE>(f1, slot<Q, LBM_LAYOUT_SOA>(n, tgt, cell + off.at(0, cy, cz)),
+                     lds[(k++) * nx + x], a.scale);
+    }
+  } else {
+#pragma unroll
+    for (int s = 0; s < Q; s++)
+      st_nt<STORE>(f1, slot<Q, LBM_LAYOUT_SOA>(n, s, cell), d[s], a.scale);
+  }
+}
+
+template <int Q, int STORE>
+constexpr size_t staged_lds_bytes(int nx) {
+  return (size_t)n_xshift<Q>() * nx * sizeof(typename Storage<STORE>::C);
 }
 
 using DenseLaunchFn = void (*)(dim3 grid, dim3 block, hipStream_t stream, void* f0, void* f1,
                                const DenseArgs& a);
 
-template <int Q, int STORE, int LAYOUT, int KIND, bool MASK>
+template <int Q, int STORE, int LAYOUT, int KIND, bool MASK, bool FORCE>
 void launch_dense(dim3 grid, dim3 block, hipStream_t stream, void* f0, void* f1,
                   const DenseArgs& a) {
   using T = typename Storage<STORE>::T;
-  hipLaunchKernelGGL((k_dense<Q, STORE, LAYOUT, KIND, MASK>), grid, block, 0, stream,
+  hipLaunchKernelGGL((k_dense<Q, STORE, LAYOUT, KIND, MASK, FORCE>), grid, block, 0, stream,
                      (const T*)f0, (T*)f1, a);
 }
 
-template <int Q, int STORE, int LAYOUT>
-DenseLaunchFn pick_kind(int kind, bool mask) {
+template <int Q, int STORE, int KIND, bool FORCE>
+void launch_staged(dim3 grid, dim3 block, hipStream_t stream, void* f0, void* f1,
+                   const DenseArgs& a) {
+  using T = typename Storage<STORE>::T;
+  const size_t lds = staged_lds_bytes<Q, STORE>(a.nx);
+  hipLaunchKernelGGL((k_dense_staged<Q, STORE, KIND, FORCE>), grid, block, lds, stream,
+                     (const T*)f0, (T*)f1, a);
+}
+
+template <int Q, int STORE, int KIND>
+DenseLaunchFn pick_staged_force(bool force) {
+  return force ? launch_staged<Q, STORE, KIND, true> : launch_staged<Q, STORE, KIND, false>;
+}
+
+template <int Q, int STORE>
+DenseLaunchFn pick_staged_kind(int kind, bool force) {
   switch (kind) {
-    case K_AA_EVEN:
-      return mask ? launch_dense<Q, STORE, LAYOUT, K_AA_EVEN, true>
-                  : launch_dense<Q, STORE, LAYOUT, K_AA_EVEN, false>;
-    case K_AA_ODD:
-      return mask ? launch_dense<Q, STORE, LAYOUT, K_AA_ODD, true>
-                  : launch_dense<Q, STORE, LAYOUT, K_AA_ODD, false>;
-    case K_AB_PUSH:
-      return mask ? launch_dense<Q, STORE, LAYOUT, K_AB_PUSH, true>
-                  : launch_dense<Q, STORE, LAYOUT, K_AB_PUSH, false>;
-    case K_AB_PULL:
-      return mask ? launch_dense<Q, STORE, LAYOUT, K_AB_PULL, true>
-                  : launch_dense<Q, STORE, LAYOUT, K_AB_PULL, false>;
+    case K_AA_ODD: return pick_staged_force<Q, STORE, K_AA_ODD>(force);
+    case K_AB_PUSH: return pick_staged_force<Q, STORE, K_AB_PUSH>(force);
+    case K_AB_PULL: return pick_staged_force<Q, STORE, K_AB_PULL>(force);
+  }
+  return nullptr;
+}
+
+// LDS-staged kernel for (storage, kind), or nullptr when the staged form does
+// not apply (AA_EVEN needs no staging) or its LDS would exceed 64 KB.
+template <int Q>
+DenseLaunchFn pick_staged_q(int storage, int kind, bool force, int nx) {
+  if (kind == K_AA_EVEN) return nullptr;
+  switch (storage) {
+    case LBM_STORE_F64:
+      if (staged_lds_bytes<Q, LBM_STORE_F64>(nx) > 65536) return nullptr;
+      return pick_staged_kind<Q, LBM_STORE_F64>(kind, force);
+    case LBM_STORE_F32S:
+      if (staged_lds_bytes<Q, LBM_STORE_F32S>(nx) > 65536) return nullptr;
+      return pick_staged_kind<Q, LBM_STORE_F32S>(kind, force);
+    case LBM_STORE_F16S:
+      if (staged_lds_bytes<Q, LBM_STORE_F16S>(nx) > 65536) return nullptr;
+      return pick_staged_kind<Q, LBM_STORE_F16S>(kind, force);
+  }
+  return nullptr;
+}
+
+template <int Q, int STORE, int LAYOUT, int KIND>
+DenseLaunchFn pick_flags(bool mask, bool force) {
+  if (mask) return force ? launch_dense<Q, STORE, LAYOUT, KIND, true, true>
+                         : launch_dense<Q, STORE, LAYOUT, KIND, true, false>;
+  return force ? launch_dense<Q, STORE, LAYOUT, KIND, false, true>
+               : launch_dense<Q, STORE, LAYOUT, KIND, false, false>;
+}
+
+template <int Q, int STORE, int LAYOUT>
+DenseLaunchFn pick_kind(int kind, bool mask, bool force) {
+  switch (kind) {
+    case K_AA_EVEN: return pick_flags<Q, STORE, LAYOUT, K_AA_EVEN>(mask, force);
+    case K_AA_ODD: return pick_flags<Q, STORE, LAYOUT, K_AA_ODD>(mask, force);
+    case K_AB_PUSH: return pick_flags<Q, STORE, LAYOUT, K_AB_PUSH>(mask, force);
+    case K_AB_PULL: return pick_flags<Q, STORE, LAYOUT, K_AB_PULL>(mask, force);
   }
   return nullptr;
 }
 
 template <int Q>
-DenseLaunchFn pick_dense_q(int storage, int layout, int kind, bool mask) {
+DenseLaunchFn pick_dense_q(int storage, int layout, int kind, bool mask, bool force) {
   if (layout == LBM_LAYOUT_SOA) {
     switch (storage) {
-      case LBM_STORE_F64: return pick_kind<Q, LBM_STORE_F64, LBM_LAYOUT_SOA>(kind, mask);
-      case LBM_STORE_F32S: return pick_kind<Q, LBM_STORE_F32S, LBM_LAYOUT_SOA>(kind, mask);
-      case LBM_STORE_F16S: return pick_kind<Q, LBM_STORE_F16S, LBM_LAYOUT_SOA>(kind, mask);
+      case LBM_STORE_F64: return pick_kind<Q, LBM_STORE_F64, LBM_LAYOUT_SOA>(kind, mask, force);
+      case LBM_STORE_F32S: return pick_kind<Q, LBM_STORE_F32S, LBM_LAYOUT_SOA>(kind, mask, force);
+      case LBM_STORE_F16S: return pick_kind<Q, LBM_STORE_F16S, LBM_LAYOUT_SOA>(kind, mask, force);
     }
   } else {
     switch (storage) {
-      case LBM_STORE_F64: return pick_kind<Q, LBM_STORE_F64, LBM_LAYOUT_AOS>(kind, mask);
-      case LBM_STORE_F32S: return pick_kind<Q, LBM_STORE_F32S, LBM_LAYOUT_AOS>(kind, mask);
-      case LBM_STORE_F16S: return pick_kind<Q, LBM_STORE_F16S, LBM_LAYOUT_AOS>(kind, mask);
+      case LBM_STORE_F64: return pick_kind<Q, LBM_STORE_F64, LBM_LAYOUT_AOS>(kind, mask, force);
+      case LBM_STORE_F32S: return pick_kind<Q, LBM_STORE_F32S, LBM_LAYOUT_AOS>(kind, mask, force);
+      case LBM_STORE_F16S: return pick_kind<Q, LBM_STORE_F16S, LBM_LAYOUT_AOS>(kind, mask, force);
     }
   }
   return nullptr;
 }
 
 // Defined in lbm_dense_q19.hip and lbm_dense_q27.hip (one translation unit per Q).
-DenseLaunchFn pick_dense19(int storage, int layout, int kind, bool mask);
-DenseLaunchFn pick_dense27(int storage, int layout, int kind, bool mask);
+// force = false selects the zero-body-force kernels (only valid for F = 0).
+DenseLaunchFn pick_dense19(int storage, int layout, int kind, bool mask, bool force);
+DenseLaunchFn pick_dense27(int storage, int layout, int kind, bool mask, bool force);
+DenseLaunchFn pick_staged19(int storage, int kind, bool force, int nx);
+DenseLaunchFn pick_staged27(int storage, int kind, bool force, int nx);
 
 }  // namespace lbm

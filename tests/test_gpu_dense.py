@@ -93,6 +93,56 @@ def test_dense_f64_bitwise_vs_oracle(q, pattern, layout, masked):
         assert not got[:, solid_cells].any()
 
 
+@pytest.mark.parametrize("q", [19, 27])
+@pytest.mark.parametrize("pattern", ["aa", "push", "pull"])
+@pytest.mark.parametrize("storage", ["f64", "f32s"])
+def test_dense_zero_force_kernels(q, pattern, storage):
+    """F = 0 selects kernels without the Guo terms; for FP64 they equal the
+    oracle (which evaluates every Guo term) bit for bit."""
+    nx, ny, nz = 66, 10, 8
+    geom = O.DenseGeometry(nx, ny, nz)
+    fc = seeded_fc(geom, q, seed=8)
+    omega, force, steps = 1 / 0.55, (0.0, 0.0, 0.0), 5
+    with DenseLattice(nx, ny, nz, q=q, pattern="aa" if pattern == "aa" else "ab",
+                      pull=pattern == "pull", storage=storage, omega=omega, force=force) as lat:
+        lat.set_state(fc)
+        lat.step(steps)
+        got = lat.get_state()
+    want = oracle_run(geom, q, fc, pattern, steps, omega, force)
+    if storage == "f64":
+        assert np.array_equal(got, want)
+    else:
+        assert np.abs(got - want).max() < 1e-7
+
+
+STAGED = [(q, pat, st, force) for q in (19, 27) for pat in ("aa", "push", "pull")
+          for st in ("f64", "f32s", "f16s") for force in (True, False)]
+
+
+@pytest.mark.parametrize("q,pattern,storage,force", STAGED)
+def test_dense_staged_rows(q, pattern, storage, force):
+    """Fully periodic SOA boxes whose rows are a multiple of 64 cells run the
+    LDS-staged gather/scatter kernels (one workgroup per x-row).  FP64 equals
+    the oracle bit for bit; FP32/FP16 deviation storage within its precision."""
+    nx, ny, nz = 128, 6, 5
+    geom = O.DenseGeometry(nx, ny, nz)
+    fc = seeded_fc(geom, q, seed=12)
+    omega = 1 / 0.7
+    F = (1e-5, -2e-6, 3e-6) if force else (0.0, 0.0, 0.0)
+    steps = 5
+    with DenseLattice(nx, ny, nz, q=q, pattern="aa" if pattern == "aa" else "ab",
+                      pull=pattern == "pull", storage=storage, omega=omega, force=F) as lat:
+        lat.set_state(fc)
+        lat.step(steps)
+        got = lat.get_state()
+    want = oracle_run(geom, q, fc, pattern, steps, omega, F)
+    if storage == "f64":
+        assert np.array_equal(got, want)
+    else:
+        tol = 1e-7 if storage == "f32s" else 2e-5
+        assert np.abs(got - want).max() < tol
+
+
 def test_dense_config1_tgv_64cubed_1000_steps():
     """D3Q19 BGK, AA, SOA, FP64, 64^3 fully periodic, tau = 0.6, rho0 = 1,
     Taylor-Green u0 = 0.05 + seeded U(-1e-4, 1e-4) noise, 1000 steps, with the
