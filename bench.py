@@ -47,6 +47,7 @@ def parse():
     ap.add_argument("--q", type=int, default=19)
     ap.add_argument("--legs", default="f64,f16s,f32s,f64_force")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     return ap.parse_args()
 
@@ -98,6 +99,32 @@ def gpu_leg(leg, n, q, steps, warmup):
         "n_fluid": nf,
         "mass_drift": (m1 - m0) / m0,
     }
+
+
+def adjoint_leg(n=128, steps=500, every=50, storage="f64"):
+    """Adjoint MLUPS on the adjoint configuration: D3Q19 n^3 channel (walls
+    y = 0, n-1), Brinkman field gamma = smoothed seeded noise (sigma 4, seed 4)
+    in [0, 1], F = (1e-6, 0, 0), tau = 0.8, J = kinetic energy after `steps`
+    steps, checkpoint every `every`.  Value = fluid nodes * steps / backward
+    device time (recomputation included)."""
+    from lbmminiapp_amd import capi
+
+    solid = capi.channel(n, n, n)
+    gamma = capi.smooth_random_field(n, n, n, 4.0, 4)
+    nf = int((solid == 0).sum())
+    with capi.Adjoint(n, n, n, q=19, storage=storage, omega=1 / 0.8, force=(1e-6, 0, 0),
+                      solid=solid, gamma=gamma, objective="kinetic_energy",
+                      checkpoint_every=every) as adj:
+        t0 = time.perf_counter()
+        J = adj.forward(steps)
+        fwd_s = time.perf_counter() - t0
+        _, _, ms = adj.backward(tau=False)
+        mem = adj.device_bytes()
+    return {"value": round(nf * steps / (ms * 1e3), 1), "unit": "MLUPS", "storage": storage,
+            "backward_ms": round(ms, 2), "forward_ms": round(fwd_s * 1e3, 2),
+            "objective": J, "device_bytes": mem,
+            "config": f"D3Q19 {n}^3 channel, Brinkman gamma field, {steps} steps, "
+                      f"checkpoint every {every}, {storage}"}
 
 
 def cpu_baseline(budget_s):
@@ -208,6 +235,11 @@ def main():
             "achieved_gbs": round(r["achieved_gbs"], 1), "frac": round(r["frac"], 4),
             "bytes_per_lup": r["bytes_per_lup"], "speedup_vs_f64": round(r["mlups"] / head["mlups"], 3),
             "mass_drift": r["mass_drift"]}
+    if not args.no_adjoint:
+        line["adjoint"] = adjoint_leg(storage="f64")
+        line["adjoint_f32"] = adjoint_leg(storage="f32s")
+        print(f"# adjoint: {line['adjoint']['value']} MLUPS (FP64), "
+              f"{line['adjoint_f32']['value']} MLUPS (FP32)", file=sys.stderr, flush=True)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(line), flush=True)

@@ -171,6 +171,54 @@ lbm_status lbm_dense_moments(lbm_dense* h, double* rho, double* u);
 lbm_status lbm_dense_mass(lbm_dense* h, double* mass);
 lbm_status lbm_dense_synchronize(lbm_dense* h);
 
+/* ---------------------------------------------------------------- discrete adjoint
+ * No counterpart in the reference.  Gradients of an objective of the final
+ * state after T collide-stream steps with respect to the per-cell Brinkman
+ * field gamma and relaxation-time field tau.  Forward model: the dense
+ * engine's AB-push step (reference BGK + Guo arithmetic, halfway bounce-back)
+ * with omega = 1/tau(x) and u = (1 - gamma(x)) (m + F/2) / rho.  Objectives:
+ * 0 = sum_x |m|^2 / (2 rho) (kinetic energy), 1 = sum over the plane x = outlet_x
+ * of m_x (outlet flux).  The forward pass stores a checkpoint every
+ * `checkpoint_every` steps; the backward pass recomputes one segment at a time
+ * (every step is computed twice) and runs the exact transpose of each step. */
+typedef struct lbm_adjoint lbm_adjoint;
+
+typedef struct {
+  int32_t nx, ny, nz;
+  int32_t q;                 /* 19 | 27 */
+  int32_t storage;           /* LBM_STORE_F64 (FP64 forward + adjoint) | LBM_STORE_F32S (FP32) */
+  int32_t periodic[3];
+  double omega;              /* used where tau is NULL */
+  double force[3];
+  const uint8_t* solid;      /* host mask or NULL */
+  const double* tau;         /* host per-cell tau or NULL (then no tau gradient) */
+  const double* gamma;       /* host per-cell gamma or NULL (then no gamma gradient) */
+  int32_t objective;         /* 0 kinetic energy, 1 outlet flux */
+  int32_t outlet_x;          /* outlet plane for objective 1 */
+  int32_t checkpoint_every;  /* k >= 1 */
+} lbm_adjoint_config;
+
+lbm_status lbm_adjoint_create(const lbm_adjoint_config* cfg, lbm_adjoint** out);
+lbm_status lbm_adjoint_destroy(lbm_adjoint* h);
+/* Initial canonical state (q * ncell doubles); default is the rest state. */
+lbm_status lbm_adjoint_set_initial_state(lbm_adjoint* h, const double* f);
+/* Forward pass of nsteps steps with checkpoints; returns J (FP64 reduction). */
+lbm_status lbm_adjoint_forward(lbm_adjoint* h, int32_t nsteps, double* objective);
+/* Backward pass over the last forward pass; host gradients (ncell doubles each,
+ * NULL to skip) and the device time of the pass including recomputation. */
+lbm_status lbm_adjoint_backward(lbm_adjoint* h, double* grad_gamma, double* grad_tau,
+                                double* elapsed_ms);
+/* Adjoint state lambda^0 = dJ/df^0 after a backward pass (q * ncell doubles). */
+lbm_status lbm_adjoint_get_lambda0(lbm_adjoint* h, double* lambda);
+/* Final state f^T of the last forward pass (canonical, q * ncell doubles). */
+lbm_status lbm_adjoint_get_final_state(lbm_adjoint* h, double* f);
+/* Fourth-order central finite difference dJ/dparam at one cell (param 0 = gamma,
+ * 1 = tau), (8 [J(+h) - J(-h)] - [J(+2h) - J(-2h)]) / 12h, each difference
+ * summed node by node on the device.  Invalidates the last forward pass. */
+lbm_status lbm_adjoint_fd(lbm_adjoint* h, int32_t param, int64_t cell, double step,
+                          int32_t nsteps, double* derivative);
+lbm_status lbm_adjoint_device_bytes(lbm_adjoint* h, size_t* bytes);
+
 /* ---------------------------------------------------------------- geometry and fields
  * Host-side generators (no device work).  Replace Geometry::setupGrid
  * (src/geometries/Geometry.cpp:100-151), whose only case is the cylinder.

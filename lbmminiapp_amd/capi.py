@@ -33,6 +33,9 @@ EXPORTED = [
     "lbm_dense_run_timed", "lbm_dense_moments", "lbm_dense_mass", "lbm_dense_synchronize",
     "lbm_geometry_cylinder", "lbm_geometry_channel", "lbm_geometry_add_spheres",
     "lbm_field_smooth_random", "lbm_copy_bandwidth",
+    "lbm_adjoint_create", "lbm_adjoint_destroy", "lbm_adjoint_set_initial_state",
+    "lbm_adjoint_forward", "lbm_adjoint_backward", "lbm_adjoint_get_lambda0",
+    "lbm_adjoint_get_final_state", "lbm_adjoint_fd", "lbm_adjoint_device_bytes",
 ]
 
 
@@ -57,6 +60,16 @@ class _DenseConfig(ctypes.Structure):
                 ("force", ctypes.c_double * 3), ("solid", ctypes.c_void_p),
                 ("tau", ctypes.c_void_p), ("gamma", ctypes.c_void_p),
                 ("f16_scale", ctypes.c_double)]
+
+
+class _AdjointConfig(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("q", ctypes.c_int32), ("storage", ctypes.c_int32),
+                ("periodic", ctypes.c_int32 * 3), ("omega", ctypes.c_double),
+                ("force", ctypes.c_double * 3), ("solid", ctypes.c_void_p),
+                ("tau", ctypes.c_void_p), ("gamma", ctypes.c_void_p),
+                ("objective", ctypes.c_int32), ("outlet_x", ctypes.c_int32),
+                ("checkpoint_every", ctypes.c_int32)]
 
 
 class _DenseInfo(ctypes.Structure):
@@ -97,6 +110,11 @@ def load(path: str = LIB_PATH):
         "lbm_geometry_cylinder": [D, P, P], "lbm_geometry_channel": [I32, I32, I32, P],
         "lbm_geometry_add_spheres": [I32, I32, I32, P, D, D, D, ctypes.c_uint64, P, P],
         "lbm_field_smooth_random": [I32, I32, I32, D, ctypes.c_uint64, P],
+        "lbm_adjoint_create": [P, P], "lbm_adjoint_destroy": [P],
+        "lbm_adjoint_set_initial_state": [P, P], "lbm_adjoint_forward": [P, I32, P],
+        "lbm_adjoint_backward": [P, P, P, P], "lbm_adjoint_get_lambda0": [P, P],
+        "lbm_adjoint_get_final_state": [P, P], "lbm_adjoint_fd": [P, I32, I64, D, I32, P],
+        "lbm_adjoint_device_bytes": [P, P],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -322,6 +340,85 @@ class DenseLattice:
     def close(self):
         if self.h:
             check(load().lbm_dense_destroy(self.h))
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# --------------------------------------------------------------------------- adjoint
+class Adjoint:
+    """Discrete adjoint forward/backward pair (lbm_adjoint_*)."""
+
+    OBJECTIVES = {"kinetic_energy": 0, "outlet_flux": 1}
+
+    def __init__(self, nx, ny, nz, q=19, storage="f64", periodic=(1, 1, 1), omega=1.0,
+                 force=(0.0, 0.0, 0.0), solid=None, tau=None, gamma=None,
+                 objective="kinetic_energy", outlet_x=0, checkpoint_every=50):
+        self.shape = (int(nx), int(ny), int(nz))
+        self.q = q
+        self.ncell = int(nx) * int(ny) * int(nz)
+        self._solid = None if solid is None else np.ascontiguousarray(solid, np.uint8).reshape(-1)
+        self._tau = None if tau is None else np.ascontiguousarray(tau, np.float64).reshape(-1)
+        self._gamma = None if gamma is None else np.ascontiguousarray(gamma, np.float64).reshape(-1)
+        obj = self.OBJECTIVES[objective] if isinstance(objective, str) else int(objective)
+        cfg = _AdjointConfig(nx, ny, nz, q, STORAGE_NAMES[storage] if isinstance(storage, str) else storage,
+                             (ctypes.c_int32 * 3)(*[int(p) for p in periodic]), float(omega),
+                             (ctypes.c_double * 3)(*force), _p(self._solid), _p(self._tau),
+                             _p(self._gamma), obj, int(outlet_x), int(checkpoint_every))
+        self.h = ctypes.c_void_p()
+        check(load().lbm_adjoint_create(ctypes.byref(cfg), ctypes.byref(self.h)))
+
+    def set_initial_state(self, f):
+        f = np.ascontiguousarray(f, np.float64).reshape(-1)
+        assert f.size == self.q * self.ncell
+        check(load().lbm_adjoint_set_initial_state(self.h, _p(f)))
+
+    def forward(self, nsteps) -> float:
+        j = ctypes.c_double(0)
+        check(load().lbm_adjoint_forward(self.h, int(nsteps), ctypes.byref(j)))
+        return j.value
+
+    def backward(self, gamma=True, tau=True):
+        gg = np.empty(self.ncell) if (gamma and self._gamma is not None) else None
+        gt = np.empty(self.ncell) if (tau and self._tau is not None) else None
+        ms = ctypes.c_double(0)
+        check(load().lbm_adjoint_backward(self.h, _p(gg), _p(gt), ctypes.byref(ms)))
+        return gg, gt, ms.value
+
+    def lambda0(self):
+        out = np.empty(self.q * self.ncell)
+        check(load().lbm_adjoint_get_lambda0(self.h, _p(out)))
+        return out.reshape(self.q, self.ncell)
+
+    def final_state(self):
+        out = np.empty(self.q * self.ncell)
+        check(load().lbm_adjoint_get_final_state(self.h, _p(out)))
+        return out.reshape(self.q, self.ncell)
+
+    def finite_difference(self, param, cell, step, nsteps) -> float:
+        d = ctypes.c_double(0)
+        p = {"gamma": 0, "tau": 1}[param] if isinstance(param, str) else int(param)
+        check(load().lbm_adjoint_fd(self.h, p, int(cell), float(step), int(nsteps), ctypes.byref(d)))
+        return d.value
+
+    def device_bytes(self) -> int:
+        b = ctypes.c_size_t(0)
+        check(load().lbm_adjoint_device_bytes(self.h, ctypes.byref(b)))
+        return b.value
+
+    def close(self):
+        if self.h:
+            check(load().lbm_adjoint_destroy(self.h))
             self.h = ctypes.c_void_p()
 
     def __enter__(self):

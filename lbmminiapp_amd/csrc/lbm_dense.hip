@@ -14,7 +14,7 @@
 #include <cstdlib>
 #include <vector>
 
-#include "lbm_dense_kernels.hpp"
+#include "lbm_dense_internal.hpp"
 
 namespace lbm {
 
@@ -281,25 +281,8 @@ LBM_UTIL_KERNEL(k_mass_partial)
 
 using namespace lbm;
 
-struct lbm_dense {
-  int nx = 0, ny = 0, nz = 0, q = 19, pattern = 1, pull = 0, layout = 0, storage = 0;
-  int periodic[3] = {1, 1, 1};
-  int64_t ncell = 0, nfluid = 0, t = 0;
-  size_t esize = 8;
-  void* f[2] = {nullptr, nullptr};
-  int cur = 0;
-  uint32_t* links = nullptr;
-  double* omega_field = nullptr;
-  double* gamma_field = nullptr;
-  double* partial = nullptr;  // mass partial sums + result
-  int npartial = 1024;
-  hipStream_t stream = nullptr;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  DenseArgs args{};
-  size_t device_bytes = 0;
-};
 
-namespace {
+namespace lbm {
 
 // Utility kernels: 256-wide x segments.
 dim3 block_for(int nx) {
@@ -423,7 +406,7 @@ lbm_status init_from(lbm_dense* h, int src_kind, const double* rho_d, const doub
   return LBM_OK;
 }
 
-}  // namespace
+}  // namespace lbm
 
 extern "C" {
 

@@ -1,0 +1,34 @@
+// lbm_dense_internal.hpp -- the dense engine handle and the host helpers that
+// the adjoint module (lbm_adjoint.hip) shares with lbm_dense.hip.
+#pragma once
+#include "lbm_dense_kernels.hpp"
+
+struct lbm_dense {
+  int nx = 0, ny = 0, nz = 0, q = 19, pattern = 1, pull = 0, layout = 0, storage = 0;
+  int periodic[3] = {1, 1, 1};
+  int64_t ncell = 0, nfluid = 0, t = 0;
+  size_t esize = 8;
+  void* f[2] = {nullptr, nullptr};
+  int cur = 0;
+  uint32_t* links = nullptr;
+  double* omega_field = nullptr;
+  double* gamma_field = nullptr;
+  double* partial = nullptr;  // mass partial sums + result
+  int npartial = 1024;
+  hipStream_t stream = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  lbm::DenseArgs args{};
+  size_t device_bytes = 0;
+};
+
+namespace lbm {
+
+// launch shapes of the utility kernels: 256-wide x segments per (y, z) row
+dim3 block_for(int nx);
+dim3 grid_for(const lbm_dense* h);
+// one time-step launch of `kind` from f0 to f1 (f0 == f1 for AA), staged when possible
+lbm_status launch_kind(lbm_dense* h, int kind, void* f0, void* f1);
+// device allocation accounted in h->device_bytes
+lbm_status dmalloc(lbm_dense* h, void** p, size_t bytes, const char* what);
+
+}  // namespace lbm

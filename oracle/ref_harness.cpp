@@ -16,9 +16,11 @@
 
 #include <chrono>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 
 #include "Geometry.hh"
+#include "GeometryAAAB.hh"
 #include "KernelAA_AOS_Base.hh"
 #include "KernelAA_SOA_Base.hh"
 #include "KernelAA_SOA_Unroll.hh"
@@ -135,6 +137,39 @@ int ref_lattice(int* dirs, int* opp, double* w) {
   }
   delete lattice;
   return _LATTICESIZE_;
+}
+
+// The reference's own geometry pipeline on a deck file (src/Main.cpp:63-72):
+// GeometryAAAB -> decomposeDomain -> setupGrid -> setup -> assignFluidIndices
+// -> setupAdjacency, for one MPI task.  With pts == NULL only the fluid count
+// and the border/bulk ranges are returned; otherwise pts (n x 3) and adj
+// (n x Q, the layout the library was compiled for: SOA) are filled.
+int ref_geometry(const char* deck, int* nfluid, int* ranges, int* pts, int* adj) {
+  ensure_mpi();
+  Parameters params;
+  params.readInput(deck);
+  Lattice* lattice = nullptr;
+  if (_LATTICESIZE_ == 19) lattice = new LatticeD3Q19;
+  else lattice = new LatticeD3Q27;
+  GeometryAAAB geom(*lattice, params);
+  geom.decomposeDomain();
+  geom.setupGrid();
+  geom.setup();
+  geom.assignFluidIndices();
+  geom.setupAdjacency();
+  const int n = geom.getNumFluidPts();
+  *nfluid = n;
+  ranges[0] = geom.getBorderStart();
+  ranges[1] = geom.getBorderCount();
+  ranges[2] = geom.getBulkStart();
+  ranges[3] = geom.getBulkCount();
+  if (pts) {
+    for (int f = 0; f < n; f++)
+      for (int d = 0; d < 3; d++) pts[3 * f + d] = geom.fluidPts_[f][d];
+    std::memcpy(adj, geom.adjacency_, sizeof(int) * (size_t)n * _LATTICESIZE_);
+  }
+  delete lattice;
+  return 0;
 }
 
 }  // extern "C"

@@ -59,7 +59,38 @@ CASES = [
 ]
 
 
+def ref_geometry_fixture(q, deck_text, name):
+    """Fluid points, border/bulk ranges and adjacency from the reference's own
+    GeometryAAAB pipeline (oracle/ref_harness.cpp: ref_geometry) on a deck."""
+    import ctypes
+    import tempfile
+    L = O.ref_lib(q, "f64", 1.0)
+    L.ref_geometry.argtypes = [ctypes.c_char_p] + [ctypes.c_void_p] * 4
+    with tempfile.NamedTemporaryFile("w", suffix=".txt", delete=False) as fh:
+        fh.write(deck_text)
+        deck = fh.name
+    n = np.zeros(1, np.int32)
+    ranges = np.zeros(4, np.int32)
+    L.ref_geometry(deck.encode(), O._ptr(n), O._ptr(ranges), None, None)
+    pts = np.zeros(3 * int(n[0]), np.int32)
+    adj = np.zeros(q * int(n[0]), np.int32)
+    L.ref_geometry(deck.encode(), O._ptr(n), O._ptr(ranges), O._ptr(pts), O._ptr(adj))
+    os.unlink(deck)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), q=q, deck=deck_text, ranges=ranges,
+                        fluid_pts=pts.reshape(-1, 3), adjacency=adj)
+    print(name, int(n[0]), "fluid nodes")
+
+
+GEOMETRY_CASES = [
+    ("geometry_cylinder_s0p5_aa_q19", 19, "simulationSize 0.5\npropPattern 1\npull 0\n"),
+    ("geometry_cylinder_s0p5_ab_q19", 19, "simulationSize 0.5\npropPattern 0\npull 0\n"),
+    ("geometry_cylinder_s0p5_pull_q27", 27, "simulationSize 0.5\npropPattern 0\npull 1\n"),
+]
+
+
 def main():
+    for name, q, deck in GEOMETRY_CASES:
+        ref_geometry_fixture(q, deck, name)
     for k, (name, q, prec, omega, variant, layout, periodic, steps) in enumerate(CASES):
         if not O.ref_available(q, prec, omega):
             raise SystemExit(f"reference build for q={q} {prec} omega={omega} missing: make -C oracle")
