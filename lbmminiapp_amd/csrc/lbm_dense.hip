@@ -340,15 +340,27 @@ DenseLaunchFn pick_staged(const lbm_dense* h, int kind) {
                     : pick_staged27(h->storage, kind, force, h->nx);
 }
 
+// LBM_DENSE_BLOCKS_PER_CU=k (tuning) requests 160 KiB / k of LDS per
+// time-step workgroup, so that at most k workgroups share a CU.
+size_t step_lds_min() {
+  static const size_t bytes = [] {
+    const char* e = std::getenv("LBM_DENSE_BLOCKS_PER_CU");
+    const int k = e ? std::atoi(e) : 0;
+    return k > 0 ? (size_t)(160 * 1024 / k) & ~(size_t)1023 : (size_t)0;
+  }();
+  return bytes;
+}
+
 lbm_status launch_kind(lbm_dense* h, int kind, void* f0, void* f1) {
+  const size_t lds_min = step_lds_min();
   if (DenseLaunchFn st = pick_staged(h, kind)) {
-    st(dim3(1, h->ny, h->nz), dim3(h->nx), h->stream, f0, f1, h->args);
+    st(dim3(1, h->ny, h->nz), dim3(h->nx), h->stream, f0, f1, h->args, lds_min);
   } else {
     DenseLaunchFn fn = pick_step(h, kind);
     LBM_REQUIRE(fn, LBM_ERR_UNSUPPORTED, "no kernel for this configuration");
     const dim3 b = step_block_for(h->nx, h->storage);
     const dim3 g((h->nx + b.x - 1) / b.x, h->ny, h->nz);
-    fn(g, b, h->stream, f0, f1, h->args);
+    fn(g, b, h->stream, f0, f1, h->args, lds_min);
   }
   LBM_LAUNCH_CHECK();
   return LBM_OK;

@@ -314,22 +314,26 @@ constexpr size_t staged_lds_bytes(int nx) {
   return (size_t)n_xshift<Q>() * nx * sizeof(typename Storage<STORE>::C);
 }
 
+// lds_min: dynamic LDS to request per workgroup at least (0 = what the kernel
+// needs).  A larger request caps the resident workgroups per CU, which bounds
+// the number of concurrently open population streams per CU.
 using DenseLaunchFn = void (*)(dim3 grid, dim3 block, hipStream_t stream, void* f0, void* f1,
-                               const DenseArgs& a);
+                               const DenseArgs& a, size_t lds_min);
 
 template <int Q, int STORE, int LAYOUT, int KIND, bool MASK, bool FORCE>
 void launch_dense(dim3 grid, dim3 block, hipStream_t stream, void* f0, void* f1,
-                  const DenseArgs& a) {
+                  const DenseArgs& a, size_t lds_min) {
   using T = typename Storage<STORE>::T;
-  hipLaunchKernelGGL((k_dense<Q, STORE, LAYOUT, KIND, MASK, FORCE>), grid, block, 0, stream,
+  hipLaunchKernelGGL((k_dense<Q, STORE, LAYOUT, KIND, MASK, FORCE>), grid, block, lds_min, stream,
                      (const T*)f0, (T*)f1, a);
 }
 
 template <int Q, int STORE, int KIND, bool FORCE>
 void launch_staged(dim3 grid, dim3 block, hipStream_t stream, void* f0, void* f1,
-                   const DenseArgs& a) {
+                   const DenseArgs& a, size_t lds_min) {
   using T = typename Storage<STORE>::T;
-  const size_t lds = staged_lds_bytes<Q, STORE>(a.nx);
+  size_t lds = staged_lds_bytes<Q, STORE>(a.nx);
+  if (lds < lds_min) lds = lds_min;
   hipLaunchKernelGGL((k_dense_staged<Q, STORE, KIND, FORCE>), grid, block, lds, stream,
                      (const T*)f0, (T*)f1, a);
 }
