@@ -180,7 +180,14 @@ lbm_status lbm_dense_synchronize(lbm_dense* h);
  * 0 = sum_x |m|^2 / (2 rho) (kinetic energy), 1 = sum over the plane x = outlet_x
  * of m_x (outlet flux).  The forward pass stores a checkpoint every
  * `checkpoint_every` steps; the backward pass recomputes one segment at a time
- * (every step is computed twice) and runs the exact transpose of each step. */
+ * and runs the exact transpose of each step.  Nested levels bound the memory
+ * for long runs: with checkpoint_inner = {k2, k3} a segment of k1 steps is
+ * recomputed once storing a sub-checkpoint every k2 steps, each sub-segment
+ * likewise every k3 steps, and only the innermost segment is held whole.
+ * States held besides the two adjoint arrays: ceil(T/k1) + (ceil(k1/k2) - 1)
+ * + (ceil(k2/k3) - 1) + (k_last - 1), at least 2 for the last term when nested
+ * (unused levels drop out); each level costs one more forward pass.  The
+ * gradients do not depend on the intervals (bit for bit). */
 typedef struct lbm_adjoint lbm_adjoint;
 
 typedef struct {
@@ -195,7 +202,8 @@ typedef struct {
   const double* gamma;       /* host per-cell gamma or NULL (then no gamma gradient) */
   int32_t objective;         /* 0 kinetic energy, 1 outlet flux */
   int32_t outlet_x;          /* outlet plane for objective 1 */
-  int32_t checkpoint_every;  /* k >= 1 */
+  int32_t checkpoint_every;  /* k1 >= 1 */
+  int32_t checkpoint_inner[2]; /* {k2, k3}: 0 = level unused; k1 >= k2 >= k3 >= 1 where used */
 } lbm_adjoint_config;
 
 lbm_status lbm_adjoint_create(const lbm_adjoint_config* cfg, lbm_adjoint** out);

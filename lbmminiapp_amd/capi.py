@@ -69,7 +69,7 @@ class _AdjointConfig(ctypes.Structure):
                 ("force", ctypes.c_double * 3), ("solid", ctypes.c_void_p),
                 ("tau", ctypes.c_void_p), ("gamma", ctypes.c_void_p),
                 ("objective", ctypes.c_int32), ("outlet_x", ctypes.c_int32),
-                ("checkpoint_every", ctypes.c_int32)]
+                ("checkpoint_every", ctypes.c_int32), ("checkpoint_inner", ctypes.c_int32 * 2)]
 
 
 class _DenseInfo(ctypes.Structure):
@@ -363,7 +363,8 @@ class Adjoint:
 
     def __init__(self, nx, ny, nz, q=19, storage="f64", periodic=(1, 1, 1), omega=1.0,
                  force=(0.0, 0.0, 0.0), solid=None, tau=None, gamma=None,
-                 objective="kinetic_energy", outlet_x=0, checkpoint_every=50):
+                 objective="kinetic_energy", outlet_x=0, checkpoint_every=50,
+                 checkpoint_inner=(0, 0)):
         self.shape = (int(nx), int(ny), int(nz))
         self.q = q
         self.ncell = int(nx) * int(ny) * int(nz)
@@ -374,7 +375,8 @@ class Adjoint:
         cfg = _AdjointConfig(nx, ny, nz, q, STORAGE_NAMES[storage] if isinstance(storage, str) else storage,
                              (ctypes.c_int32 * 3)(*[int(p) for p in periodic]), float(omega),
                              (ctypes.c_double * 3)(*force), _p(self._solid), _p(self._tau),
-                             _p(self._gamma), obj, int(outlet_x), int(checkpoint_every))
+                             _p(self._gamma), obj, int(outlet_x), int(checkpoint_every),
+                             (ctypes.c_int32 * 2)(*[int(k) for k in checkpoint_inner]))
         self.h = ctypes.c_void_p()
         check(load().lbm_adjoint_create(ctypes.byref(cfg), ctypes.byref(self.h)))
 

@@ -230,8 +230,10 @@ using namespace lbm;
 struct lbm_adjoint {
   lbm_dense* d = nullptr;  // AB push engine: stream, links, fields, two state arrays
   int objective = 0, outlet_x = 0, every = 1;
+  std::vector<int64_t> inner;  // nested checkpoint intervals below `every` (k2, k3)
   int64_t T = -1;          // steps of the last forward pass
   std::vector<void*> ckpt;
+  std::vector<std::vector<void*>> level;  // level[l]: sub-checkpoints of an inner[l] level
   void* seg = nullptr;
   int64_t seg_states = 0;
   void* lam[2] = {nullptr, nullptr};
@@ -323,7 +325,25 @@ lbm_status ensure_buffers(lbm_adjoint* h, int64_t T) {
     if (lbm_status st = dmalloc(d, &p, h->state_bytes, "adjoint checkpoint")) return st;
     h->ckpt.push_back(p);
   }
-  const int64_t need_seg = std::min<int64_t>(h->every, T) - 1;
+  // A segment of an outer level spans at most `outer` steps; the level below
+  // keeps its states at offsets k, 2k, ... inside it (the start is the outer
+  // level's own state).
+  int64_t outer = std::min<int64_t>(h->every, T);
+  h->level.resize(h->inner.size());
+  for (size_t l = 0; l < h->inner.size(); l++) {
+    const int64_t k = std::min<int64_t>(h->inner[l], outer);
+    const int64_t need = (outer + k - 1) / k - 1;
+    while ((int64_t)h->level[l].size() < need) {
+      void* p = nullptr;
+      if (lbm_status st = dmalloc(d, &p, h->state_bytes, "adjoint sub-checkpoint")) return st;
+      h->level[l].push_back(p);
+    }
+    outer = k;
+  }
+  // innermost segment, held whole; with nested levels it also serves as the
+  // two-state scratch for the recomputation between sub-checkpoints
+  int64_t need_seg = outer - 1;
+  if (!h->inner.empty()) need_seg = std::max<int64_t>(need_seg, 2);
   if (need_seg > h->seg_states) {
     if (h->seg) {
       (void)hipFree(h->seg);
@@ -347,9 +367,53 @@ lbm_status set_param(lbm_adjoint* h, int param, int64_t cell, double value) {
   return LBM_OK;
 }
 
+// Adjoint steps over the `len` steps that start from state `base`, walking
+// the checkpoint levels from `l` inwards; lam[cur] holds the adjoint state at
+// the end of the range on entry and at its start on return.
+lbm_status backward_range(lbm_adjoint* h, const void* base, int64_t len, size_t l, int& cur) {
+  lbm_dense* d = h->d;
+  if (l == h->inner.size()) {
+    // innermost: recompute states 1 .. len - 1 into the segment buffer
+    const void* src = base;
+    for (int64_t j = 1; j < len; j++) {
+      void* dst = (char*)h->seg + (j - 1) * h->state_bytes;
+      if (lbm_status st = launch_kind(d, K_AB_PUSH, const_cast<void*>(src), dst)) return st;
+      src = dst;
+    }
+    const dim3 g = adj_grid(d), b = adj_block(d);
+    const bool mask = d->links != nullptr;
+    for (int64_t j = len - 1; j >= 0; j--) {
+      const void* state = j == 0 ? base : (char*)h->seg + (j - 1) * h->state_bytes;
+      launch_qs<step_u>(d, g, b, mask, state, (const void*)h->lam[cur], h->lam[1 - cur],
+                        h->grad[0], h->grad[1], d->args);
+      LBM_LAUNCH_CHECK();
+      cur = 1 - cur;
+    }
+    return LBM_OK;
+  }
+  const int64_t k = h->inner[l];
+  const int64_t nsub = (len + k - 1) / k;
+  std::vector<void*>& sub = h->level[l];
+  // recompute once, keeping the states at offsets k, 2k, ..., (nsub - 1) k;
+  // the states in between go to two scratch slots of the segment buffer
+  const void* src = base;
+  for (int64_t t = 1; t <= (nsub - 1) * k; t++) {
+    void* dst = t % k == 0 ? sub[t / k - 1] : (char*)h->seg + (t % 2) * h->state_bytes;
+    if (lbm_status st = launch_kind(d, K_AB_PUSH, const_cast<void*>(src), dst)) return st;
+    src = dst;
+  }
+  for (int64_t j = nsub - 1; j >= 0; j--) {
+    const void* start = j == 0 ? base : sub[j - 1];
+    if (lbm_status st = backward_range(h, start, std::min(k, len - j * k), l + 1, cur)) return st;
+  }
+  return LBM_OK;
+}
+
 void free_adjoint(lbm_adjoint* h) {
   if (!h) return;
   for (void* p : h->ckpt) (void)hipFree(p);
+  for (auto& lv : h->level)
+    for (void* p : lv) (void)hipFree(p);
   if (h->seg) (void)hipFree(h->seg);
   for (int i = 0; i < 2; i++) {
     if (h->lam[i]) (void)hipFree(h->lam[i]);
@@ -377,6 +441,12 @@ lbm_status lbm_adjoint_create(const lbm_adjoint_config* cfg, lbm_adjoint** out) 
               "lbm_adjoint_create: checkpoint_every must be >= 1");
   LBM_REQUIRE(cfg->objective == 0 || (cfg->outlet_x >= 0 && cfg->outlet_x < cfg->nx),
               LBM_ERR_ARG, "lbm_adjoint_create: outlet_x outside [0, nx)");
+  const int32_t k2 = cfg->checkpoint_inner[0], k3 = cfg->checkpoint_inner[1];
+  LBM_REQUIRE(k2 >= 0 && k3 >= 0 && (k2 > 0 || k3 == 0) && k2 <= cfg->checkpoint_every &&
+                  k3 <= (k2 > 0 ? k2 : 0),
+              LBM_ERR_ARG,
+              "lbm_adjoint_create: checkpoint_inner must be {0, 0}, {k2, 0} or {k2, k3} with "
+              "checkpoint_every >= k2 >= k3 >= 1");
   lbm_dense_config dc{};
   dc.nx = cfg->nx;
   dc.ny = cfg->ny;
@@ -403,6 +473,8 @@ lbm_status lbm_adjoint_create(const lbm_adjoint_config* cfg, lbm_adjoint** out) 
   h->objective = cfg->objective;
   h->outlet_x = cfg->outlet_x;
   h->every = cfg->checkpoint_every;
+  if (k2 > 0) h->inner.push_back(k2);
+  if (k3 > 0) h->inner.push_back(k3);
   h->state_bytes = (size_t)d->ncell * d->q * d->esize;
   h->lam_bytes = (size_t)d->ncell * d->q * (d->storage == LBM_STORE_F64 ? 8 : 4);
   if (cfg->tau) h->tau_host.assign(cfg->tau, cfg->tau + d->ncell);
@@ -480,25 +552,10 @@ lbm_status lbm_adjoint_backward(lbm_adjoint* h, double* grad_gamma, double* grad
     if (h->grad[i]) LBM_HIP_CHECK(hipMemsetAsync(h->grad[i], 0, sizeof(double) * d->ncell, d->stream));
   const int64_t T = h->T;
   const int64_t nseg = (T + h->every - 1) / h->every;
-  const dim3 g = adj_grid(d), b = adj_block(d);
-  const bool mask = d->links != nullptr;
   for (int64_t i = nseg - 1; i >= 0; i--) {
     const int64_t t0 = i * h->every;
     const int64_t len = std::min<int64_t>(h->every, T - t0);
-    // recompute states t0 + 1 .. t0 + len - 1 of the segment
-    void* src = h->ckpt[i];
-    for (int64_t j = 1; j < len; j++) {
-      void* dst = (char*)h->seg + (j - 1) * h->state_bytes;
-      if (lbm_status st = launch_kind(d, K_AB_PUSH, src, dst)) return st;
-      src = dst;
-    }
-    for (int64_t j = len - 1; j >= 0; j--) {
-      const void* state = j == 0 ? h->ckpt[i] : (char*)h->seg + (j - 1) * h->state_bytes;
-      launch_qs<step_u>(d, g, b, mask, state, (const void*)h->lam[cur], h->lam[1 - cur],
-                        h->grad[0], h->grad[1], d->args);
-      LBM_LAUNCH_CHECK();
-      cur = 1 - cur;
-    }
+    if (lbm_status st = backward_range(h, h->ckpt[i], len, 0, cur)) return st;
   }
   LBM_HIP_CHECK(hipEventRecord(d->e1, d->stream));
   LBM_HIP_CHECK(hipEventSynchronize(d->e1));

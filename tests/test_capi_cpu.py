@@ -77,3 +77,19 @@ def test_null_handles_are_rejected(lib):
     assert lib.lbm_dense_mass(None, None) == capi.ERR_ARG
     assert lib.lbm_sparse_aa_even(None, None, 0, 1) == capi.ERR_ARG
     assert lib.lbm_copy_bandwidth(15, 1, None) == capi.ERR_ARG
+
+
+def test_adjoint_create_rejects_bad_checkpoint_plans(lib):
+    h = ctypes.c_void_p()
+    base = dict(nx=8, ny=8, nz=8, q=19, storage=capi.STORE_F64,
+                periodic=(ctypes.c_int32 * 3)(1, 1, 1), omega=1.0,
+                force=(ctypes.c_double * 3)(0, 0, 0), solid=None, tau=None, gamma=None,
+                objective=0, outlet_x=0, checkpoint_every=10)
+    for bad in (dict(checkpoint_every=0), dict(checkpoint_inner=(ctypes.c_int32 * 2)(11, 0)),
+                dict(checkpoint_inner=(ctypes.c_int32 * 2)(0, 3)),
+                dict(checkpoint_inner=(ctypes.c_int32 * 2)(4, 5)),
+                dict(checkpoint_inner=(ctypes.c_int32 * 2)(-1, 0)),
+                dict(objective=1, outlet_x=8), dict(storage=capi.STORE_F16S)):
+        cfg = capi._AdjointConfig(**{**base, **bad})
+        assert lib.lbm_adjoint_create(ctypes.byref(cfg), ctypes.byref(h)) == capi.ERR_ARG, bad
+        assert lib.lbm_last_error()

@@ -62,12 +62,19 @@ def test_adjoint_matches_oracle(q, objective):
 
 
 def test_checkpoint_interval_does_not_change_gradients():
+    """One level (k = 1, 3, 5, 17) and nested levels ((k1, k2, k3) with ragged
+    last segments, k2 = 1, and intervals longer than the run) give the same
+    gradients bit for bit."""
     q = 19
     g, solid, gamma, tau, fc, full = small_channel(q, seed=5)
     res = []
-    for every in (1, 3, 5, 17):
+    plans = [(1, (0, 0)), (3, (0, 0)), (5, (0, 0)), (17, (0, 0)), (17, (5, 2)), (8, (3, 0)),
+             (7, (1, 0)), (16, (4, 4)), (40, (20, 3))]
+    for every, inner in plans:
+        print(f"[adjoint] checkpoints every {every}, inner {inner}", flush=True)
         with Adjoint(g.nx, g.ny, g.nz, q=q, storage="f64", omega=1.0, force=(1e-5, 0, 0),
-                     solid=solid, tau=tau, gamma=gamma, checkpoint_every=every) as adj:
+                     solid=solid, tau=tau, gamma=gamma, checkpoint_every=every,
+                     checkpoint_inner=inner) as adj:
             adj.set_initial_state(full)
             adj.forward(17)
             res.append(adj.backward()[:2])
