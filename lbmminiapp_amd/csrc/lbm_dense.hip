@@ -325,15 +325,18 @@ DenseLaunchFn pick_step(const lbm_dense* h, int kind) {
 }
 
 // The LDS-staged kernels apply to fully periodic SOA boxes whose rows fit one
-// workgroup (nx a multiple of 64, nx <= kDenseMaxBlock).  LBM_DENSE_STAGED=0
-// forces the unstaged kernels (tuning and cross-checks).
+// workgroup (nx a multiple of 64, nx <= kDenseMaxBlock), in FP64 and FP32
+// deviation storage (with 2-byte FP16 elements the unstaged kernel measured
+// faster: 2.1 vs 2.2 ms per 512^3 D3Q19 odd step).  LBM_DENSE_STAGED=0 forces
+// the unstaged kernels everywhere, LBM_DENSE_STAGED=2 also stages FP16 (tuning
+// and cross-checks).
 DenseLaunchFn pick_staged(const lbm_dense* h, int kind) {
-  static const bool enabled = [] {
+  static const int mode = [] {
     const char* e = std::getenv("LBM_DENSE_STAGED");
-    return !(e && e[0] == '0');
+    return e ? std::atoi(e) : 1;
   }();
-  if (!enabled || h->links || h->layout != LBM_LAYOUT_SOA || h->nx % 64 != 0 ||
-      h->nx > kDenseMaxBlock)
+  if (mode == 0 || h->links || h->layout != LBM_LAYOUT_SOA || h->nx % 64 != 0 ||
+      h->nx > kDenseMaxBlock || (h->storage == LBM_STORE_F16S && mode < 2))
     return nullptr;
   const bool force = h->args.F0 != 0.0 || h->args.F1 != 0.0 || h->args.F2 != 0.0;
   return h->q == 19 ? pick_staged19(h->storage, kind, force, h->nx)

@@ -84,3 +84,64 @@ def test_dense_engine_on_reference_cylinder(tmp_path):
     sparse_canon = O.aa_canonical(sp["f"], q, n, O.SOA, sp["adj"], 200)
     dense = de["f"].reshape(q, -1)
     assert np.array_equal(dense[:, cell], sparse_canon)
+
+
+def read_vtk(path):
+    """Legacy VTK (ASCII or BINARY) as written by the driver: dims and the rho,
+    id, velocity and nodes arrays, read in file order."""
+    raw = open(path, "rb").read()
+    pos = 0
+
+    def line():
+        nonlocal pos
+        end = raw.index(b"\n", pos)
+        text = raw[pos:end]
+        pos = end + 1
+        return text
+
+    hdr = [line() for _ in range(8)]
+    binary = hdr[2].strip() == b"BINARY"
+    dims = tuple(int(v) for v in hdr[4].split()[1:4])
+    npts = int(np.prod(dims))
+    assert hdr[7].split()[1] == str(npts).encode()
+    out = {"dims": dims, "binary": binary}
+    for name, comp, dt in (("rho", 1, "f8"), ("id", 1, "i4"), ("velocity", 3, "f8"),
+                           ("nodes", 3, "i4")):
+        h = line()
+        assert h.split()[1].decode() == name, h
+        if h.startswith(b"SCALARS"):
+            assert line().startswith(b"LOOKUP_TABLE")
+        if binary:
+            nbytes = npts * comp * int(dt[1])
+            arr = np.frombuffer(raw[pos:pos + nbytes], ">" + dt)
+            pos += nbytes
+            assert line() == b""
+        else:
+            arr = np.array(b" ".join(line() for _ in range(npts)).split(), dtype=dt)
+        out[name] = arr.reshape(npts, comp).squeeze()
+    return out
+
+
+def test_vtk_binary_matches_ascii(tmp_path):
+    """writevtk 2 (legacy binary) holds the same fields as writevtk 1 (the
+    reference's ASCII format): ids and coordinates equal, rho and velocity
+    equal to the 6 significant digits the ASCII stream prints."""
+    base = "simulationSize 1\npropPattern 1\npull 0\nunroll 1\ntimesteps 50\n"
+    res = {}
+    for mode in (1, 2):
+        d = tmp_path / f"vtk{mode}"
+        d.mkdir()
+        (d / "deck.txt").write_text(base + f"writevtk {mode}\n")
+        print(f"[driver] writevtk {mode}", flush=True)
+        r = subprocess.run([DRIVER, "deck.txt"], cwd=d, capture_output=True, text=True,
+                           timeout=240)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        res[mode] = read_vtk(d / "outs.00000.vtk")
+    a, b = res[1], res[2]
+    assert not a["binary"] and b["binary"] and a["dims"] == b["dims"]
+    assert np.array_equal(a["id"], b["id"]) and np.array_equal(a["nodes"], b["nodes"])
+    fluid = b["id"] >= 0
+    assert fluid.sum() > 0 and np.abs(b["rho"][fluid] - 1).max() < 1e-2
+    assert np.allclose(a["rho"], b["rho"], rtol=1e-5, atol=0)
+    assert np.allclose(a["velocity"], b["velocity"], rtol=1e-5, atol=1e-12)
+    assert np.abs(b["velocity"][fluid, 0]).max() > 0
