@@ -220,11 +220,13 @@ lbm_status lbm_adjoint_backward(lbm_adjoint* h, double* grad_gamma, double* grad
 lbm_status lbm_adjoint_get_lambda0(lbm_adjoint* h, double* lambda);
 /* Final state f^T of the last forward pass (canonical, q * ncell doubles). */
 lbm_status lbm_adjoint_get_final_state(lbm_adjoint* h, double* f);
-/* Fourth-order central finite difference dJ/dparam at one cell (param 0 = gamma,
- * 1 = tau), (8 [J(+h) - J(-h)] - [J(+2h) - J(-2h)]) / 12h, each difference
- * summed node by node on the device.  Invalidates the last forward pass. */
+/* Central finite difference dJ/dparam at one cell (param 0 = gamma, 1 = tau) of
+ * order 2, 4 or 6 from D_k = J(p + k h) - J(p - k h), each D_k summed node by
+ * node on the device: D_1 / 2h; (8 D_1 - D_2) / 12h; (45 D_1 - 9 D_2 + D_3) / 60h.
+ * The round-off of D_k divided by h sets the usable step from below, the
+ * truncation (h^order) from above.  Invalidates the last forward pass. */
 lbm_status lbm_adjoint_fd(lbm_adjoint* h, int32_t param, int64_t cell, double step,
-                          int32_t nsteps, double* derivative);
+                          int32_t nsteps, int32_t order, double* derivative);
 lbm_status lbm_adjoint_device_bytes(lbm_adjoint* h, size_t* bytes);
 
 /* ---------------------------------------------------------------- geometry and fields

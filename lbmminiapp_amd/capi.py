@@ -113,7 +113,7 @@ def load(path: str = LIB_PATH):
         "lbm_adjoint_create": [P, P], "lbm_adjoint_destroy": [P],
         "lbm_adjoint_set_initial_state": [P, P], "lbm_adjoint_forward": [P, I32, P],
         "lbm_adjoint_backward": [P, P, P, P], "lbm_adjoint_get_lambda0": [P, P],
-        "lbm_adjoint_get_final_state": [P, P], "lbm_adjoint_fd": [P, I32, I64, D, I32, P],
+        "lbm_adjoint_get_final_state": [P, P], "lbm_adjoint_fd": [P, I32, I64, D, I32, I32, P],
         "lbm_adjoint_device_bytes": [P, P],
     }
     for name, args in sig.items():
@@ -407,10 +407,11 @@ class Adjoint:
         check(load().lbm_adjoint_get_final_state(self.h, _p(out)))
         return out.reshape(self.q, self.ncell)
 
-    def finite_difference(self, param, cell, step, nsteps) -> float:
+    def finite_difference(self, param, cell, step, nsteps, order=6) -> float:
         d = ctypes.c_double(0)
         p = {"gamma": 0, "tau": 1}[param] if isinstance(param, str) else int(param)
-        check(load().lbm_adjoint_fd(self.h, p, int(cell), float(step), int(nsteps), ctypes.byref(d)))
+        check(load().lbm_adjoint_fd(self.h, p, int(cell), float(step), int(nsteps), int(order),
+                                    ctypes.byref(d)))
         return d.value
 
     def device_bytes(self) -> int:

@@ -465,7 +465,7 @@ lbm_status lbm_adjoint_create(const lbm_adjoint_config* cfg, lbm_adjoint** out) 
   dc.tau = cfg->tau;
   dc.gamma = cfg->gamma;
   lbm_adjoint* h = new lbm_adjoint;
-  if (lbm_status st = lbm_dense_create(&dc, &h->d)) {
+  if (lbm_status st = dense_create(&dc, &h->d, false)) {
     delete h;
     return st;
   }
@@ -599,9 +599,11 @@ lbm_status lbm_adjoint_get_final_state(lbm_adjoint* h, double* f) {
 }
 
 lbm_status lbm_adjoint_fd(lbm_adjoint* h, int32_t param, int64_t cell, double step,
-                          int32_t nsteps, double* derivative) {
+                          int32_t nsteps, int32_t order, double* derivative) {
   LBM_REQUIRE(h && derivative && nsteps >= 1 && step > 0.0, LBM_ERR_ARG,
               "lbm_adjoint_fd: bad arguments");
+  LBM_REQUIRE(order == 2 || order == 4 || order == 6, LBM_ERR_ARG,
+              "lbm_adjoint_fd: order must be 2, 4 or 6");
   LBM_REQUIRE(param == 0 || param == 1, LBM_ERR_ARG, "lbm_adjoint_fd: param 0 = gamma, 1 = tau");
   lbm_dense* d = h->d;
   LBM_REQUIRE(cell >= 0 && cell < d->ncell, LBM_ERR_ARG, "lbm_adjoint_fd: cell out of range");
@@ -611,11 +613,13 @@ lbm_status lbm_adjoint_fd(lbm_adjoint* h, int32_t param, int64_t cell, double st
     if (!h->fd[i])
       if (lbm_status st = dmalloc(d, &h->fd[i], h->state_bytes, "finite-difference state")) return st;
   const double p0 = param == 0 ? h->gamma_host[cell] : h->tau_host[cell];
-  double dj[2] = {0.0, 0.0};
-  const double deltas[2] = {step, 2.0 * step};
-  for (int k = 0; k < 2; k++) {
+  // D_k = J(p + k h) - J(p - k h), k = 1 .. order / 2, each summed node by node
+  double dj[3] = {0.0, 0.0, 0.0};
+  const int npairs = order / 2;
+  for (int k = 0; k < npairs; k++) {
+    const double delta = (k + 1) * step;
     for (int sgn = 0; sgn < 2; sgn++) {
-      const double v = sgn == 0 ? p0 + deltas[k] : p0 - deltas[k];
+      const double v = sgn == 0 ? p0 + delta : p0 - delta;
       if (lbm_status st = set_param(h, param, cell, v)) return st;
       void* fin = nullptr;
       if (lbm_status st = run_forward(h, nsteps, false, &fin)) return st;
@@ -630,7 +634,10 @@ lbm_status lbm_adjoint_fd(lbm_adjoint* h, int32_t param, int64_t cell, double st
   if (lbm_status st = set_param(h, param, cell, p0)) return st;
   h->final_state = nullptr;  // the engine's arrays were reused
   h->T = -1;
-  *derivative = (8.0 * dj[0] - dj[1]) / (12.0 * step);
+  // central-difference weights of orders 2, 4 and 6
+  if (order == 2) *derivative = dj[0] / (2.0 * step);
+  else if (order == 4) *derivative = (8.0 * dj[0] - dj[1]) / (12.0 * step);
+  else *derivative = (45.0 * dj[0] - 9.0 * dj[1] + dj[2]) / (60.0 * step);
   return LBM_OK;
 }
 

@@ -52,11 +52,9 @@ __global__ void __launch_bounds__(256)
   const int y = blockIdx.y, z = blockIdx.z;
   if (x >= nx) return;
   const int64_t cell = x + (int64_t)nx * (y + (int64_t)ny * z);
-  if (solid[cell]) {
-    links[cell] = SOLID_BIT;
-    return;
-  }
-  uint32_t m = 0;
+  // bit s: x + c_s is solid or outside; solid cells carry the same bits (they
+  // place the zeros of the whole-line stores) plus SOLID_BIT
+  uint32_t m = solid[cell] ? SOLID_BIT : 0u;
   for (int s = 0; s < Q; s++) {
     int p[3] = {x + Lattice<Q>::c(s, 0), y + Lattice<Q>::c(s, 1), z + Lattice<Q>::c(s, 2)};
     const int dims[3] = {nx, ny, nz};
@@ -296,7 +294,10 @@ dim3 grid_for(const lbm_dense* h) {
 }
 
 // Time-step kernels: a whole x-row per workgroup when it fits (kDenseMaxBlock),
-// else 256-wide segments.  LBM_DENSE_BLOCK_X overrides the width (tuning).
+// else 256-wide segments.  FP32 deviation storage measured faster with 256-wide
+// segments when they tile the row exactly (D3Q19 AA at nx = 512: 35.6k vs 31.8k
+// MLUPS); otherwise whole rows (porous push at nx = 384: 2.50 vs 2.95 ms/step).
+// LBM_DENSE_BLOCK_X overrides the width (tuning).
 dim3 step_block_for(int nx, int storage) {
   static const int forced = [] {
     const char* e = std::getenv("LBM_DENSE_BLOCK_X");
@@ -304,7 +305,8 @@ dim3 step_block_for(int nx, int storage) {
   }();
   int bx;
   if (forced > 0) bx = forced;
-  else if (nx <= kDenseMaxBlock && storage != LBM_STORE_F32S) bx = ((nx + 63) / 64) * 64;
+  else if (nx <= kDenseMaxBlock && !(storage == LBM_STORE_F32S && nx % 256 == 0))
+    bx = ((nx + 63) / 64) * 64;
   else bx = 256;
   if (bx > kDenseMaxBlock) bx = kDenseMaxBlock;
   if (bx < 64) bx = 64;
@@ -421,11 +423,7 @@ lbm_status init_from(lbm_dense* h, int src_kind, const double* rho_d, const doub
   return LBM_OK;
 }
 
-}  // namespace lbm
-
-extern "C" {
-
-lbm_status lbm_dense_create(const lbm_dense_config* cfg, lbm_dense** out) {
+lbm_status dense_create(const lbm_dense_config* cfg, lbm_dense** out, bool push_by_gather) {
   LBM_REQUIRE(cfg && out, LBM_ERR_ARG, "lbm_dense_create: null pointer");
   *out = nullptr;
   LBM_REQUIRE(cfg->nx > 0 && cfg->ny > 0 && cfg->nz > 0, LBM_ERR_ARG,
@@ -464,6 +462,18 @@ lbm_status lbm_dense_create(const lbm_dense_config* cfg, lbm_dense** out) {
   }
   const bool need_links =
       cfg->solid != nullptr || !h->periodic[0] || !h->periodic[1] || !h->periodic[2];
+  // AB push on a masked SOA grid runs the pull (gather) kernel one streaming
+  // step out of phase: with P_0 = S^-1 f_0, (C S)^T P_0 = S^-1 (S C)^T f_0, so
+  // the push state f_T is the gather of the pull storage, which is how every
+  // readout of a pull handle reads it.  Same per-node arithmetic, bit-identical
+  // states; the masked scatter measured 14.7k vs 20.4k MLUPS for the gather
+  // (D3Q19 FP32 deviations, 384^3, 35% solid).  LBM_DENSE_PUSH_GATHER=0 keeps
+  // the scatter kernel (A/B timing).
+  if (push_by_gather && h->pattern == LBM_PATTERN_AB && !h->pull && need_links &&
+      h->layout == LBM_LAYOUT_SOA) {
+    const char* e = std::getenv("LBM_DENSE_PUSH_GATHER");
+    if (!(e && e[0] == '0')) h->pull = 1;
+  }
   lbm_status st = LBM_OK;
   auto fail = [&](lbm_status s) {
     free_all(h);
@@ -543,6 +553,15 @@ lbm_status lbm_dense_create(const lbm_dense_config* cfg, lbm_dense** out) {
   a.ny = h->ny;
   a.nz = h->nz;
   a.ncell = h->ncell;
+  a.periodic_bits = (h->periodic[0] ? 1u : 0u) | (h->periodic[1] ? 2u : 0u) |
+                    (h->periodic[2] ? 4u : 0u);
+  // Zeros into solid cells' slots complete the SOA row segments (+12% on a 35%
+  // solid D3Q19 push step); in AOS the extra scattered stores cost more than
+  // they save (-33%).  LBM_DENSE_FILL_SOLID=0 turns it off (A/B timing).
+  {
+    const char* e = std::getenv("LBM_DENSE_FILL_SOLID");
+    a.fill_solid = h->layout == LBM_LAYOUT_SOA && !(e && e[0] == '0');
+  }
   a.links = h->links;
   a.omega_field = h->omega_field;
   a.gamma_field = h->gamma_field;
@@ -565,6 +584,14 @@ lbm_status lbm_dense_create(const lbm_dense_config* cfg, lbm_dense** out) {
   }
   *out = h;
   return LBM_OK;
+}
+
+}  // namespace lbm
+
+extern "C" {
+
+lbm_status lbm_dense_create(const lbm_dense_config* cfg, lbm_dense** out) {
+  return dense_create(cfg, out, true);
 }
 
 lbm_status lbm_dense_destroy(lbm_dense* h) {

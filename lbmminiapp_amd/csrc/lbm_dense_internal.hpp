@@ -30,5 +30,9 @@ dim3 grid_for(const lbm_dense* h);
 lbm_status launch_kind(lbm_dense* h, int kind, void* f0, void* f1);
 // device allocation accounted in h->device_bytes
 lbm_status dmalloc(lbm_dense* h, void** p, size_t bytes, const char* what);
+// lbm_dense_create; push_by_gather = false keeps AB push handles on the
+// scatter kernel and the push storage convention (the adjoint drives the
+// push kernel on the handle's arrays itself)
+lbm_status dense_create(const lbm_dense_config* cfg, lbm_dense** out, bool push_by_gather);
 
 }  // namespace lbm

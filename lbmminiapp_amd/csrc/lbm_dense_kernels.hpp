@@ -40,6 +40,8 @@ enum DenseKind { K_AA_EVEN = 0, K_AA_ODD = 1, K_AB_PUSH = 2, K_AB_PULL = 3 };
 struct DenseArgs {
   int nx, ny, nz;
   int64_t ncell;
+  uint32_t periodic_bits;      // bit k: dimension k is periodic
+  int fill_solid;              // write zeros into the slots of solid cells (whole-line stores)
   const uint32_t* links;       // nullptr when every dimension is periodic and nothing is solid
   const double* omega_field;   // per-cell omega = 1/tau, or nullptr
   const double* gamma_field;   // per-cell Brinkman gamma, or nullptr
@@ -123,11 +125,31 @@ struct Offsets {
   }
 };
 
+// true when x + c leaves the box through a non-periodic face
+__device__ __forceinline__ bool leaves_box(const DenseArgs& a, int x, int y, int z, int cx, int cy,
+                                           int cz) {
+  const uint32_t p = a.periodic_bits;
+  return (!(p & 1u) && ((cx < 0 && x == 0) || (cx > 0 && x == a.nx - 1))) ||
+         (!(p & 2u) && ((cy < 0 && y == 0) || (cy > 0 && y == a.ny - 1))) ||
+         (!(p & 4u) && ((cz < 0 && z == 0) || (cz > 0 && z == a.nz - 1)));
+}
+
 // Blocks are one x-row segment of up to kDenseMaxBlock cells (a full row when
 // nx <= kDenseMaxBlock, so that every cache line shared by x-shifted accesses is
 // used by one workgroup).
 constexpr int kDenseMaxBlock = 512;
 
+// Solid cells (MASK): their slots are never read.  With a.fill_solid every
+// (slot, cell) of the box is still stored exactly once per step, zeros going
+// to solid cells, so that no cache line is left partly written (a partly
+// written line costs the memory a read-modify-write):
+//   local-write kinds: a solid cell stores zeros into its own slots;
+//   scatter kinds, direction s, target T = x + c_s:
+//     T fluid          fluid x streams; solid x stores nothing
+//     T solid (inside) fluid x bounces back, and either x stores a zero into
+//                      the slot at T that streaming would have written
+//     T outside        fluid x bounces back; solid x stores a zero into that
+//                      bounce-back slot
 template <int Q, int STORE, int LAYOUT, int KIND, bool MASK, bool FORCE>
 __global__ void __launch_bounds__(kDenseMaxBlock)
     k_dense(const typename Storage<STORE>::T* f0, typename Storage<STORE>::T* f1, DenseArgs a) {
@@ -137,64 +159,76 @@ __global__ void __launch_bounds__(kDenseMaxBlock)
   const int y = blockIdx.y, z = blockIdx.z;
   const int64_t cell = x + (int64_t)a.nx * (y + (int64_t)a.ny * z);
   uint32_t lk = 0;
+  bool solid = false;
   if constexpr (MASK) {
     lk = a.links[cell];
-    if (lk & SOLID_BIT) return;
+    solid = (lk & SOLID_BIT) != 0;
+    if (solid && !a.fill_solid) return;
   }
   const Offsets off(x, y, z, a.nx, a.ny, a.nz);
   const int64_t n = a.ncell;
 
   C d[Q];
+  if (!solid) {
 #pragma unroll
-  for (int s = 0; s < Q; s++) {
-    const int o = Lattice<Q>::opp(s);
-    const int cx = Lattice<Q>::c(s, 0), cy = Lattice<Q>::c(s, 1), cz = Lattice<Q>::c(s, 2);
-    int64_t idx;
-    // Row-aligned accesses stream past the caches; x-shifted ones (c_x != 0 in
-    // the gathering kinds) share their boundary cache line with the
-    // neighbouring wavefront and must stay cacheable.
-    bool aligned = true;
-    if constexpr (KIND == K_AA_EVEN) {
-      idx = slot<Q, LAYOUT>(n, o, cell);
-    } else if constexpr (KIND == K_AB_PUSH) {
-      idx = slot<Q, LAYOUT>(n, s, cell);
-    } else {
-      aligned = cx == 0;
-      if (MASK && ((lk >> o) & 1u)) idx = slot<Q, LAYOUT>(n, o, cell);
-      else idx = slot<Q, LAYOUT>(n, s, cell + off.at(-cx, -cy, -cz));
+    for (int s = 0; s < Q; s++) {
+      const int o = Lattice<Q>::opp(s);
+      const int cx = Lattice<Q>::c(s, 0), cy = Lattice<Q>::c(s, 1), cz = Lattice<Q>::c(s, 2);
+      int64_t idx;
+      // Row-aligned accesses stream past the caches; x-shifted ones (c_x != 0 in
+      // the gathering kinds) share their boundary cache line with the
+      // neighbouring wavefront and must stay cacheable.
+      bool aligned = true;
+      if constexpr (KIND == K_AA_EVEN) {
+        idx = slot<Q, LAYOUT>(n, o, cell);
+      } else if constexpr (KIND == K_AB_PUSH) {
+        idx = slot<Q, LAYOUT>(n, s, cell);
+      } else {
+        aligned = cx == 0;
+        if (MASK && ((lk >> o) & 1u)) idx = slot<Q, LAYOUT>(n, o, cell);
+        else idx = slot<Q, LAYOUT>(n, s, cell + off.at(-cx, -cy, -cz));
+      }
+      if (aligned && LAYOUT == LBM_LAYOUT_SOA) d[s] = ld_nt<STORE>(f0, idx, a.inv_scale);
+      else d[s] = ld<STORE>(f0, idx, a.inv_scale);
     }
-    if (aligned && LAYOUT == LBM_LAYOUT_SOA) d[s] = ld_nt<STORE>(f0, idx, a.inv_scale);
-    else d[s] = ld<STORE>(f0, idx, a.inv_scale);
+
+    const double om = a.omega_field ? a.omega_field[cell] : a.omega;
+    const double F[3] = {a.F0, a.F1, a.F2};
+    if constexpr (STORE == LBM_STORE_F64) {
+      if (a.gamma_field) collide_ref<Q, double, true, FORCE>(d, d, om, F, a.gamma_field[cell]);
+      else collide_ref<Q, double, false, FORCE>(d, d, om, F, 0.0);
+    } else {
+      if (a.gamma_field) collide_shifted<Q, true, FORCE>(d, om, F, a.gamma_field[cell]);
+      else collide_shifted<Q, false, FORCE>(d, om, F, 0.0);
+    }
   }
 
-  const double om = a.omega_field ? a.omega_field[cell] : a.omega;
-  const double F[3] = {a.F0, a.F1, a.F2};
-  if constexpr (STORE == LBM_STORE_F64) {
-    if (a.gamma_field) collide_ref<Q, double, true, FORCE>(d, d, om, F, a.gamma_field[cell]);
-    else collide_ref<Q, double, false, FORCE>(d, d, om, F, 0.0);
-  } else {
-    if (a.gamma_field) collide_shifted<Q, true, FORCE>(d, om, F, a.gamma_field[cell]);
-    else collide_shifted<Q, false, FORCE>(d, om, F, 0.0);
-  }
-
+  auto put = [&](int64_t idx, C v, bool aligned) {
+    if (aligned && LAYOUT == LBM_LAYOUT_SOA) st_nt<STORE>(f1, idx, v, a.scale);
+    else st<STORE>(f1, idx, v, a.scale);
+  };
 #pragma unroll
   for (int s = 0; s < Q; s++) {
     const int o = Lattice<Q>::opp(s);
     const int cx = Lattice<Q>::c(s, 0), cy = Lattice<Q>::c(s, 1), cz = Lattice<Q>::c(s, 2);
-    int64_t idx;
-    bool aligned = true;
     if constexpr (KIND == K_AA_EVEN || KIND == K_AB_PULL) {
-      idx = slot<Q, LAYOUT>(n, s, cell);
+      put(slot<Q, LAYOUT>(n, s, cell), solid ? C(0) : d[s], true);
     } else {
-      aligned = cx == 0;
+      const bool aligned = cx == 0;
+      // streaming target: AA odd writes slot opp(s) at x + c_s, AB push slot s
+      const int64_t sidx = slot<Q, LAYOUT>(n, KIND == K_AA_ODD ? o : s, cell + off.at(cx, cy, cz));
       const bool bb = MASK && ((lk >> s) & 1u);
-      if constexpr (KIND == K_AA_ODD)
-        idx = bb ? slot<Q, LAYOUT>(n, s, cell) : slot<Q, LAYOUT>(n, o, cell + off.at(cx, cy, cz));
-      else
-        idx = bb ? slot<Q, LAYOUT>(n, o, cell) : slot<Q, LAYOUT>(n, s, cell + off.at(cx, cy, cz));
+      if (!bb) {
+        if (!solid) put(sidx, d[s], aligned);
+      } else {
+        // halfway bounce-back slot: AA odd slot s at x, AB push slot opp(s) at x
+        const int64_t bidx = slot<Q, LAYOUT>(n, KIND == K_AA_ODD ? s : o, cell);
+        const bool out = leaves_box(a, x, y, z, cx, cy, cz);
+        if (!solid) put(bidx, d[s], aligned);
+        else if (out) put(bidx, C(0), aligned);
+        if (!out && a.fill_solid) put(sidx, C(0), aligned);
+      }
     }
-    if (aligned && LAYOUT == LBM_LAYOUT_SOA) st_nt<STORE>(f1, idx, d[s], a.scale);
-    else st<STORE>(f1, idx, d[s], a.scale);
   }
 }
 
