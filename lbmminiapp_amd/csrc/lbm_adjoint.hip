@@ -70,14 +70,17 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
   for (int s = 0; s < Q; s++) {
     const int cx = Lattice<Q>::c(s, 0), cy = Lattice<Q>::c(s, 1), cz = Lattice<Q>::c(s, 2);
-    fs[s] = full_value<Q, STORE>(ld<STORE>(f, slot<Q, LBM_LAYOUT_SOA>(n, s, cell), a.inv_scale), s);
+    // the state and the row-aligned adjoint values are read once: stream them
+    // past the caches; x-shifted reads share lines with the neighbouring wave
+    fs[s] = full_value<Q, STORE>(ld_nt<STORE>(f, slot<Q, LBM_LAYOUT_SOA>(n, s, cell), a.inv_scale), s);
     rho += fs[s];
     m0 += cx * fs[s];
     m1 += cy * fs[s];
     m2 += cz * fs[s];
     const bool bb = MASK && ((lk >> s) & 1u);
-    lg[s] = bb ? lam_in[slot<Q, LBM_LAYOUT_SOA>(n, Lattice<Q>::opp(s), cell)]
-               : lam_in[slot<Q, LBM_LAYOUT_SOA>(n, s, cell + off.at(cx, cy, cz))];
+    const int64_t li = bb ? slot<Q, LBM_LAYOUT_SOA>(n, Lattice<Q>::opp(s), cell)
+                          : slot<Q, LBM_LAYOUT_SOA>(n, s, cell + off.at(cx, cy, cz));
+    lg[s] = cx == 0 ? __builtin_nontemporal_load(lam_in + li) : lam_in[li];
   }
   const double om = a.omega_field ? a.omega_field[cell] : a.omega;
   const double gam = a.gamma_field ? a.gamma_field[cell] : 0.0;
@@ -113,9 +116,9 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
   for (int k = 0; k < Q; k++) {
     const R c0 = Lattice<Q>::c(k, 0), c1 = Lattice<Q>::c(k, 1), c2 = Lattice<Q>::c(k, 2);
-    lam_out[slot<Q, LBM_LAYOUT_SOA>(n, k, cell)] =
-        (R(1) - omr) * lg[k] + aa +
-        (B0 * (sc * c0 - u0) + B1 * (sc * c1 - u1) + B2 * (sc * c2 - u2)) / rr;
+    const R v = (R(1) - omr) * lg[k] + aa +
+                (B0 * (sc * c0 - u0) + B1 * (sc * c1 - u1) + B2 * (sc * c2 - u2)) / rr;
+    __builtin_nontemporal_store(v, lam_out + slot<Q, LBM_LAYOUT_SOA>(n, k, cell));
   }
   if (grad_gamma) grad_gamma[cell] += -((double)B0 * us0 + (double)B1 * us1 + (double)B2 * us2);
   if (grad_tau) grad_tau[cell] += -om * om * (double)dw;

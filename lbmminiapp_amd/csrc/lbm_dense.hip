@@ -326,23 +326,24 @@ DenseLaunchFn pick_step(const lbm_dense* h, int kind) {
                     : pick_dense27(h->storage, h->layout, kind, mask, force);
 }
 
-// The LDS-staged kernels apply to fully periodic SOA boxes whose rows fit one
-// workgroup (nx a multiple of 64, nx <= kDenseMaxBlock), in FP64 and FP32
-// deviation storage (with 2-byte FP16 elements the unstaged kernel measured
-// faster: 2.1 vs 2.2 ms per 512^3 D3Q19 odd step).  LBM_DENSE_STAGED=0 forces
-// the unstaged kernels everywhere, LBM_DENSE_STAGED=2 also stages FP16 (tuning
-// and cross-checks).
+// The LDS-staged kernels cover SOA grids whose rows fit one workgroup (nx a
+// multiple of 64, nx <= kDenseMaxBlock): every kind on fully periodic boxes,
+// and the pull gather also with solid cells and walls.  By default (staged
+// mode 1, set at create from LBM_DENSE_STAGED) they run on periodic boxes in
+// FP64 and FP32 deviation storage only, where they measured faster; the
+// unstaged kernels measured faster for FP16 rows (2.1 vs 2.2 ms per 512^3
+// D3Q19 odd step) and for masked gathers (35% solid porous pull at 384^3:
+// 20.4k vs 18.7k MLUPS; a 5% solid channel: equal).  Mode 0 never stages,
+// mode 2 stages every case covered (tuning and cross-checks).
 DenseLaunchFn pick_staged(const lbm_dense* h, int kind) {
-  static const int mode = [] {
-    const char* e = std::getenv("LBM_DENSE_STAGED");
-    return e ? std::atoi(e) : 1;
-  }();
-  if (mode == 0 || h->links || h->layout != LBM_LAYOUT_SOA || h->nx % 64 != 0 ||
-      h->nx > kDenseMaxBlock || (h->storage == LBM_STORE_F16S && mode < 2))
+  const int mode = h->staged_mode;
+  const bool mask = h->links != nullptr;
+  if (mode == 0 || h->layout != LBM_LAYOUT_SOA || h->nx % 64 != 0 ||
+      h->nx > kDenseMaxBlock || (mode < 2 && (h->storage == LBM_STORE_F16S || mask)))
     return nullptr;
   const bool force = h->args.F0 != 0.0 || h->args.F1 != 0.0 || h->args.F2 != 0.0;
-  return h->q == 19 ? pick_staged19(h->storage, kind, force, h->nx)
-                    : pick_staged27(h->storage, kind, force, h->nx);
+  return h->q == 19 ? pick_staged19(h->storage, kind, force, h->nx, mask)
+                    : pick_staged27(h->storage, kind, force, h->nx, mask);
 }
 
 // LBM_DENSE_BLOCKS_PER_CU=k (tuning) requests 160 KiB / k of LDS per
@@ -451,6 +452,7 @@ lbm_status dense_create(const lbm_dense_config* cfg, lbm_dense** out, bool push_
   h->layout = cfg->layout;
   h->storage = cfg->storage;
   for (int k = 0; k < 3; k++) h->periodic[k] = cfg->periodic[k] ? 1 : 0;
+  if (const char* e = std::getenv("LBM_DENSE_STAGED")) h->staged_mode = std::atoi(e);
   h->ncell = (int64_t)cfg->nx * cfg->ny * cfg->nz;
   h->esize = cfg->storage == LBM_STORE_F64 ? 8 : (cfg->storage == LBM_STORE_F32S ? 4 : 2);
   // fluid count (host mask)

@@ -6,6 +6,7 @@
 struct lbm_dense {
   int nx = 0, ny = 0, nz = 0, q = 19, pattern = 1, pull = 0, layout = 0, storage = 0;
   int periodic[3] = {1, 1, 1};
+  int staged_mode = 1;  // LDS-staged kernels: 0 never, 1 default cases, 2 every case they cover
   int64_t ncell = 0, nfluid = 0, t = 0;
   size_t esize = 8;
   void* f[2] = {nullptr, nullptr};

@@ -251,10 +251,16 @@ __host__ __device__ constexpr int n_xshift() {
 //   scatter: node x writes its result to LDS[(x + cx) mod nx], thread x then
 //            stores LDS[x] as element x of row (y+cy, z+cz)
 // Same reads, writes and arithmetic as k_dense, so the results are identical.
-template <int Q, int STORE, int KIND, bool FORCE>
+// MASK (AB_PULL only): every thread still loads its aligned row element (the
+// periodic offsets keep rows in bounds across non-periodic faces too); a node
+// whose neighbour x - c_s is solid or outside then takes slot opp(s) at x
+// instead, solid nodes skip the collision and store zeros (a.fill_solid), and
+// all threads stay for the barrier.
+template <int Q, int STORE, int KIND, bool FORCE, bool MASK = false>
 __global__ void __launch_bounds__(kDenseMaxBlock)
     k_dense_staged(const typename Storage<STORE>::T* f0, typename Storage<STORE>::T* f1,
                    DenseArgs a) {
+  static_assert(!MASK || KIND == K_AB_PULL, "masked staging covers the pull gather");
   using C = typename Storage<STORE>::C;
   constexpr int NSH = n_xshift<Q>();
   extern __shared__ unsigned char lbm_smem[];
@@ -299,14 +305,28 @@ __global__ void __launch_bounds__(kDenseMaxBlock)
     }
   }
 
-  const double om = a.omega_field ? a.omega_field[cell] : a.omega;
-  const double F[3] = {a.F0, a.F1, a.F2};
-  if constexpr (STORE == LBM_STORE_F64) {
-    if (a.gamma_field) collide_ref<Q, double, true, FORCE>(d, d, om, F, a.gamma_field[cell]);
-    else collide_ref<Q, double, false, FORCE>(d, d, om, F, 0.0);
-  } else {
-    if (a.gamma_field) collide_shifted<Q, true, FORCE>(d, om, F, a.gamma_field[cell]);
-    else collide_shifted<Q, false, FORCE>(d, om, F, 0.0);
+  uint32_t lk = 0;
+  bool solid = false;
+  if constexpr (MASK) {
+    lk = a.links[cell];
+    solid = (lk & SOLID_BIT) != 0;
+#pragma unroll
+    for (int s = 0; s < Q; s++) {
+      const int o = Lattice<Q>::opp(s);
+      if (!solid && ((lk >> o) & 1u)) d[s] = ld_nt<STORE>(f0, slot<Q, LBM_LAYOUT_SOA>(n, o, cell), a.inv_scale);
+    }
+  }
+
+  if (!solid) {
+    const double om = a.omega_field ? a.omega_field[cell] : a.omega;
+    const double F[3] = {a.F0, a.F1, a.F2};
+    if constexpr (STORE == LBM_STORE_F64) {
+      if (a.gamma_field) collide_ref<Q, double, true, FORCE>(d, d, om, F, a.gamma_field[cell]);
+      else collide_ref<Q, double, false, FORCE>(d, d, om, F, 0.0);
+    } else {
+      if (a.gamma_field) collide_shifted<Q, true, FORCE>(d, om, F, a.gamma_field[cell]);
+      else collide_shifted<Q, false, FORCE>(d, om, F, 0.0);
+    }
   }
 
   if constexpr (SCATTER) {
@@ -337,9 +357,10 @@ __global__ void __launch_bounds__(kDenseMaxBlock)
                      lds[(k++) * nx + x], a.scale);
     }
   } else {
+    if (solid && !a.fill_solid) return;
 #pragma unroll
     for (int s = 0; s < Q; s++)
-      st_nt<STORE>(f1, slot<Q, LBM_LAYOUT_SOA>(n, s, cell), d[s], a.scale);
+      st_nt<STORE>(f1, slot<Q, LBM_LAYOUT_SOA>(n, s, cell), solid ? C(0) : d[s], a.scale);
   }
 }
 
@@ -362,46 +383,49 @@ void launch_dense(dim3 grid, dim3 block, hipStream_t stream, void* f0, void* f1,
                      (const T*)f0, (T*)f1, a);
 }
 
-template <int Q, int STORE, int KIND, bool FORCE>
+template <int Q, int STORE, int KIND, bool FORCE, bool MASK>
 void launch_staged(dim3 grid, dim3 block, hipStream_t stream, void* f0, void* f1,
                    const DenseArgs& a, size_t lds_min) {
   using T = typename Storage<STORE>::T;
   size_t lds = staged_lds_bytes<Q, STORE>(a.nx);
   if (lds < lds_min) lds = lds_min;
-  hipLaunchKernelGGL((k_dense_staged<Q, STORE, KIND, FORCE>), grid, block, lds, stream,
+  hipLaunchKernelGGL((k_dense_staged<Q, STORE, KIND, FORCE, MASK>), grid, block, lds, stream,
                      (const T*)f0, (T*)f1, a);
 }
 
-template <int Q, int STORE, int KIND>
+template <int Q, int STORE, int KIND, bool MASK>
 DenseLaunchFn pick_staged_force(bool force) {
-  return force ? launch_staged<Q, STORE, KIND, true> : launch_staged<Q, STORE, KIND, false>;
+  return force ? launch_staged<Q, STORE, KIND, true, MASK>
+               : launch_staged<Q, STORE, KIND, false, MASK>;
 }
 
 template <int Q, int STORE>
-DenseLaunchFn pick_staged_kind(int kind, bool force) {
+DenseLaunchFn pick_staged_kind(int kind, bool force, bool mask) {
+  if (mask) return kind == K_AB_PULL ? pick_staged_force<Q, STORE, K_AB_PULL, true>(force) : nullptr;
   switch (kind) {
-    case K_AA_ODD: return pick_staged_force<Q, STORE, K_AA_ODD>(force);
-    case K_AB_PUSH: return pick_staged_force<Q, STORE, K_AB_PUSH>(force);
-    case K_AB_PULL: return pick_staged_force<Q, STORE, K_AB_PULL>(force);
+    case K_AA_ODD: return pick_staged_force<Q, STORE, K_AA_ODD, false>(force);
+    case K_AB_PUSH: return pick_staged_force<Q, STORE, K_AB_PUSH, false>(force);
+    case K_AB_PULL: return pick_staged_force<Q, STORE, K_AB_PULL, false>(force);
   }
   return nullptr;
 }
 
-// LDS-staged kernel for (storage, kind), or nullptr when the staged form does
-// not apply (AA_EVEN needs no staging) or its LDS would exceed 64 KB.
+// LDS-staged kernel for (storage, kind, mask), or nullptr when the staged form
+// does not apply (AA_EVEN needs no staging; with a mask only the pull gather
+// is staged) or its LDS would exceed 64 KB.
 template <int Q>
-DenseLaunchFn pick_staged_q(int storage, int kind, bool force, int nx) {
+DenseLaunchFn pick_staged_q(int storage, int kind, bool force, int nx, bool mask) {
   if (kind == K_AA_EVEN) return nullptr;
   switch (storage) {
     case LBM_STORE_F64:
       if (staged_lds_bytes<Q, LBM_STORE_F64>(nx) > 65536) return nullptr;
-      return pick_staged_kind<Q, LBM_STORE_F64>(kind, force);
+      return pick_staged_kind<Q, LBM_STORE_F64>(kind, force, mask);
     case LBM_STORE_F32S:
       if (staged_lds_bytes<Q, LBM_STORE_F32S>(nx) > 65536) return nullptr;
-      return pick_staged_kind<Q, LBM_STORE_F32S>(kind, force);
+      return pick_staged_kind<Q, LBM_STORE_F32S>(kind, force, mask);
     case LBM_STORE_F16S:
       if (staged_lds_bytes<Q, LBM_STORE_F16S>(nx) > 65536) return nullptr;
-      return pick_staged_kind<Q, LBM_STORE_F16S>(kind, force);
+      return pick_staged_kind<Q, LBM_STORE_F16S>(kind, force, mask);
   }
   return nullptr;
 }
@@ -447,7 +471,7 @@ DenseLaunchFn pick_dense_q(int storage, int layout, int kind, bool mask, bool fo
 // force = false selects the zero-body-force kernels (only valid for F = 0).
 DenseLaunchFn pick_dense19(int storage, int layout, int kind, bool mask, bool force);
 DenseLaunchFn pick_dense27(int storage, int layout, int kind, bool mask, bool force);
-DenseLaunchFn pick_staged19(int storage, int kind, bool force, int nx);
-DenseLaunchFn pick_staged27(int storage, int kind, bool force, int nx);
+DenseLaunchFn pick_staged19(int storage, int kind, bool force, int nx, bool mask);
+DenseLaunchFn pick_staged27(int storage, int kind, bool force, int nx, bool mask);
 
 }  // namespace lbm

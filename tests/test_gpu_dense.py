@@ -127,6 +127,39 @@ def test_dense_masked_kernel_variants(q, pattern, monkeypatch):
 
 
 @pytest.mark.parametrize("q", [19, 27])
+@pytest.mark.parametrize("periodic", [(1, 0, 1), (0, 1, 1), (0, 0, 0)])
+@pytest.mark.parametrize("storage", ["f64", "f32s"])
+def test_dense_masked_staged_gather(q, periodic, storage, monkeypatch):
+    """Rows of 128 cells with 30% solid cells and walls (including a
+    non-periodic x face): the pull step, and the push step run as a gather,
+    take the LDS-staged kernel with bounce-back reads (staged mode 2; the
+    default runs masked gathers unstaged).  FP64 equals the oracle bit for bit
+    with the solid fill on and off; FP32 deviations within 1e-7."""
+    monkeypatch.setenv("LBM_DENSE_STAGED", "2")
+    nx, ny, nz = 128, 10, 7
+    solid = (O.uniform01(41, np.arange(nx * ny * nz, dtype=np.uint64)) < 0.3).astype(np.uint8)
+    geom = O.DenseGeometry(nx, ny, nz, solid, periodic=periodic)
+    fc = seeded_fc(geom, q, seed=17)
+    omega, force = 1 / 0.8, (1e-6, 3e-7, -2e-7)
+    for pattern in ("pull", "push"):
+        want = oracle_run(geom, q, fc, pattern, 6, omega, force)
+        for fill in ("1", "0"):
+            monkeypatch.setenv("LBM_DENSE_FILL_SOLID", fill)
+            print(f"[dense] staged masked q={q} {periodic} {storage} {pattern} fill={fill}",
+                  flush=True)
+            with DenseLattice(nx, ny, nz, q=q, pattern="ab", pull=pattern == "pull",
+                              storage=storage, periodic=periodic, omega=omega, force=force,
+                              solid=solid) as lat:
+                lat.set_state(dense_canonical(geom, q, fc))
+                lat.step(6)
+                got = lat.get_state()[:, geom.cell_of_fluid]
+            if storage == "f64":
+                assert np.array_equal(got, want), (pattern, fill)
+            else:
+                assert np.abs(got - want).max() < 1e-7, (pattern, fill)
+
+
+@pytest.mark.parametrize("q", [19, 27])
 @pytest.mark.parametrize("pattern", ["aa", "push", "pull"])
 @pytest.mark.parametrize("storage", ["f64", "f32s"])
 def test_dense_zero_force_kernels(q, pattern, storage):
@@ -153,10 +186,12 @@ STAGED = [(q, pat, st, force) for q in (19, 27) for pat in ("aa", "push", "pull"
 
 
 @pytest.mark.parametrize("q,pattern,storage,force", STAGED)
-def test_dense_staged_rows(q, pattern, storage, force):
+def test_dense_staged_rows(q, pattern, storage, force, monkeypatch):
     """Fully periodic SOA boxes whose rows are a multiple of 64 cells run the
-    LDS-staged gather/scatter kernels (one workgroup per x-row).  FP64 equals
+    LDS-staged gather/scatter kernels (one workgroup per x-row; staged mode 2,
+    so the FP16 variants, unstaged by default, are covered too).  FP64 equals
     the oracle bit for bit; FP32/FP16 deviation storage within its precision."""
+    monkeypatch.setenv("LBM_DENSE_STAGED", "2")
     nx, ny, nz = 128, 6, 5
     geom = O.DenseGeometry(nx, ny, nz)
     fc = seeded_fc(geom, q, seed=12)
