@@ -293,24 +293,48 @@ dim3 grid_for(const lbm_dense* h) {
   return dim3((h->nx + b.x - 1) / b.x, h->ny, h->nz);
 }
 
-// Time-step kernels: a whole x-row per workgroup when it fits (kDenseMaxBlock),
-// else 256-wide segments.  FP32 deviation storage measured faster with 256-wide
-// segments when they tile the row exactly (D3Q19 AA at nx = 512: 35.6k vs 31.8k
-// MLUPS); otherwise whole rows (porous push at nx = 384: 2.50 vs 2.95 ms/step).
-// LBM_DENSE_BLOCK_X overrides the width (tuning).
-dim3 step_block_for(int nx, int storage) {
-  static const int forced = [] {
-    const char* e = std::getenv("LBM_DENSE_BLOCK_X");
-    return e ? std::atoi(e) : 0;
-  }();
-  int bx;
-  if (forced > 0) bx = forced;
-  else if (nx <= kDenseMaxBlock && !(storage == LBM_STORE_F32S && nx % 256 == 0))
-    bx = ((nx + 63) / 64) * 64;
-  else bx = 256;
-  if (bx > kDenseMaxBlock) bx = kDenseMaxBlock;
-  if (bx < 64) bx = 64;
-  return dim3(bx);
+// Default block width of the unstaged time-step kernels: a whole x-row per
+// workgroup when it fits (kDenseMaxBlock), else 256-wide segments.  FP32
+// deviation storage measured faster with 256-wide segments when they tile the
+// row exactly (D3Q19 AA at nx = 512: 35.6k vs 31.8k MLUPS); otherwise whole
+// rows (porous push at nx = 384: 2.50 vs 2.95 ms per step).
+int default_step_bx(int nx, int storage) {
+  if (nx <= kDenseMaxBlock && !(storage == LBM_STORE_F32S && nx % 256 == 0))
+    return ((nx + 63) / 64) * 64;
+  return 256;
+}
+
+// Dynamic LDS that leaves room for at most k workgroups per CU (160 KiB LDS).
+size_t lds_for_blocks_per_cu(int k) {
+  return k > 0 ? (size_t)(160 * 1024 / k) & ~(size_t)1023 : (size_t)0;
+}
+
+// Launch shapes, fixed at create.  The SOA AA even step reads and writes only
+// row-aligned slots, so it needs no whole rows; it measured fastest with
+// small workgroups at low occupancy, like the 19-stream in-place copy pattern,
+// whose DRAM efficiency drops as more streams are open per CU.  D3Q19 512^3
+// even step, best of a sweep over widths 64-512 and 0-8 workgroups per CU:
+//   FP64   128 wide, <= 4 per CU: 6.91 ms (512-wide rows: 7.86 ms)
+//   FP32   256 wide, <= 4 per CU: 3.39 ms (256 wide, no cap: 3.47 ms)
+//   FP16   256 wide, no cap:      1.85 ms (512-wide rows: 2.06 ms)  Environment overrides (tuning): LBM_DENSE_BLOCK_X
+// and LBM_DENSE_BLOCKS_PER_CU for the unstaged kernels (the latter also for
+// the staged ones), LBM_DENSE_EVEN_BLOCK_X and LBM_DENSE_EVEN_BLOCKS_PER_CU for
+// the even step.
+void set_launch_shapes(lbm_dense* h) {
+  auto env_int = [](const char* name, int dflt) {
+    const char* e = std::getenv(name);
+    return e ? std::atoi(e) : dflt;
+  };
+  auto clamp_bx = [](int bx) { return bx < 64 ? 64 : (bx > kDenseMaxBlock ? kDenseMaxBlock : bx); };
+  h->step_bx = clamp_bx(env_int("LBM_DENSE_BLOCK_X", default_step_bx(h->nx, h->storage)));
+  h->step_lds = lds_for_blocks_per_cu(env_int("LBM_DENSE_BLOCKS_PER_CU", 0));
+  int even_bx = h->step_bx, even_k = 0;
+  if (h->layout == LBM_LAYOUT_SOA) {
+    even_bx = h->storage == LBM_STORE_F64 ? 128 : 256;
+    even_k = h->storage == LBM_STORE_F16S ? 0 : 4;
+  }
+  h->even_bx = clamp_bx(env_int("LBM_DENSE_EVEN_BLOCK_X", even_bx));
+  h->even_lds = lds_for_blocks_per_cu(env_int("LBM_DENSE_EVEN_BLOCKS_PER_CU", even_k));
 }
 
 // state convention of the current array
@@ -346,27 +370,16 @@ DenseLaunchFn pick_staged(const lbm_dense* h, int kind) {
                     : pick_staged27(h->storage, kind, force, h->nx, mask);
 }
 
-// LBM_DENSE_BLOCKS_PER_CU=k (tuning) requests 160 KiB / k of LDS per
-// time-step workgroup, so that at most k workgroups share a CU.
-size_t step_lds_min() {
-  static const size_t bytes = [] {
-    const char* e = std::getenv("LBM_DENSE_BLOCKS_PER_CU");
-    const int k = e ? std::atoi(e) : 0;
-    return k > 0 ? (size_t)(160 * 1024 / k) & ~(size_t)1023 : (size_t)0;
-  }();
-  return bytes;
-}
-
 lbm_status launch_kind(lbm_dense* h, int kind, void* f0, void* f1) {
-  const size_t lds_min = step_lds_min();
   if (DenseLaunchFn st = pick_staged(h, kind)) {
-    st(dim3(1, h->ny, h->nz), dim3(h->nx), h->stream, f0, f1, h->args, lds_min);
+    st(dim3(1, h->ny, h->nz), dim3(h->nx), h->stream, f0, f1, h->args, h->step_lds);
   } else {
     DenseLaunchFn fn = pick_step(h, kind);
     LBM_REQUIRE(fn, LBM_ERR_UNSUPPORTED, "no kernel for this configuration");
-    const dim3 b = step_block_for(h->nx, h->storage);
+    const bool even = kind == K_AA_EVEN;
+    const dim3 b(even ? h->even_bx : h->step_bx);
     const dim3 g((h->nx + b.x - 1) / b.x, h->ny, h->nz);
-    fn(g, b, h->stream, f0, f1, h->args, lds_min);
+    fn(g, b, h->stream, f0, f1, h->args, even ? h->even_lds : h->step_lds);
   }
   LBM_LAUNCH_CHECK();
   return LBM_OK;
@@ -453,6 +466,7 @@ lbm_status dense_create(const lbm_dense_config* cfg, lbm_dense** out, bool push_
   h->storage = cfg->storage;
   for (int k = 0; k < 3; k++) h->periodic[k] = cfg->periodic[k] ? 1 : 0;
   if (const char* e = std::getenv("LBM_DENSE_STAGED")) h->staged_mode = std::atoi(e);
+  set_launch_shapes(h);
   h->ncell = (int64_t)cfg->nx * cfg->ny * cfg->nz;
   h->esize = cfg->storage == LBM_STORE_F64 ? 8 : (cfg->storage == LBM_STORE_F32S ? 4 : 2);
   // fluid count (host mask)

@@ -7,6 +7,11 @@ struct lbm_dense {
   int nx = 0, ny = 0, nz = 0, q = 19, pattern = 1, pull = 0, layout = 0, storage = 0;
   int periodic[3] = {1, 1, 1};
   int staged_mode = 1;  // LDS-staged kernels: 0 never, 1 default cases, 2 every case they cover
+  // launch shape of the unstaged time-step kernels (block width, dynamic LDS
+  // floor that caps workgroups per CU), and of the SOA AA even kernel, whose
+  // accesses are all row-aligned
+  int step_bx = 256, even_bx = 256;
+  size_t step_lds = 0, even_lds = 0;
   int64_t ncell = 0, nfluid = 0, t = 0;
   size_t esize = 8;
   void* f[2] = {nullptr, nullptr};
