@@ -53,7 +53,7 @@ def parse():
     return ap.parse_args()
 
 
-def gpu_leg(leg, n, q, steps, warmup):
+def gpu_leg(leg, n, q, steps, warmup, keep_fields=False):
     """leg: f64 | f32s | f16s (Taylor-Green, F = 0) or f64_force (the reference's
     body force F = (1e-6, 0, 0) switched on, same initial state)."""
     from lbmminiapp_amd.capi import DenseLattice
@@ -80,6 +80,7 @@ def gpu_leg(leg, n, q, steps, warmup):
         wall = time.perf_counter() - t0
         m1 = lat.mass()
         info = lat.info()
+        fields = lat.moments() if keep_fields else None
     finally:
         lat.close()
     nf = info["n_fluid"]
@@ -99,6 +100,8 @@ def gpu_leg(leg, n, q, steps, warmup):
         "frac": achieved / PEAK_HBM_GBS,
         "n_fluid": nf,
         "mass_drift": (m1 - m0) / m0,
+        "time_step": info["time_step"],
+        "fields": fields,
     }
 
 
@@ -257,8 +260,12 @@ def main():
     # the CPU leg first, while this process has not touched the GPU (it spawns workers)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
     results = {}
+    # density and velocity of the F = 0 legs after the same steps from the same state: the
+    # reduced-precision legs are compared with the FP64 leg (itself bit-identical to the oracle)
+    keep = "f64" in legs and any(l in ("f32s", "f16s") for l in legs)
     for leg in legs:
-        results[leg] = gpu_leg(leg, n, q, args.steps, args.warmup)
+        results[leg] = gpu_leg(leg, n, q, args.steps, args.warmup,
+                               keep_fields=keep and leg in ("f64", "f32s", "f16s"))
         print(f"# {leg}: {results[leg]['mlups']:.1f} MLUPS, "
               f"{results[leg]['achieved_gbs']:.0f} GB/s ({100 * results[leg]['frac']:.1f}% of "
               f"{PEAK_HBM_GBS:.0f} GB/s)", file=sys.stderr, flush=True)
@@ -298,6 +305,15 @@ def main():
             "achieved_gbs": round(r["achieved_gbs"], 1), "frac": round(r["frac"], 4),
             "bytes_per_lup": r["bytes_per_lup"], "speedup_vs_f64": round(r["mlups"] / head["mlups"], 3),
             "mass_drift": r["mass_drift"]}
+        if r["fields"] is not None and head["fields"] is not None:
+            import numpy as np
+            assert r["time_step"] == head["time_step"]
+            (rho, u), (rho64, u64) = r["fields"], head["fields"]
+            line[f"{leg}_storage"].update({
+                "steps_compared": int(head["time_step"]),
+                "rel_l2_u_vs_f64": float(np.linalg.norm(u - u64) / np.linalg.norm(u64)),
+                "rel_l2_rho_dev_vs_f64": float(np.linalg.norm(rho - rho64)
+                                               / np.linalg.norm(rho64 - 1.0))})
     if not args.no_adjoint:
         line["adjoint"] = adjoint_leg(storage="f64")
         line["adjoint_f32"] = adjoint_leg(storage="f32s")
