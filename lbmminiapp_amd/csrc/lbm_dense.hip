@@ -312,11 +312,14 @@ size_t lds_for_blocks_per_cu(int k) {
 // Launch shapes, fixed at create.  The SOA AA even step reads and writes only
 // row-aligned slots, so it needs no whole rows; it measured fastest with
 // small workgroups at low occupancy, like the 19-stream in-place copy pattern,
-// whose DRAM efficiency drops as more streams are open per CU.  D3Q19 512^3
-// even step, best of a sweep over widths 64-512 and 0-8 workgroups per CU:
-//   FP64   128 wide, <= 4 per CU: 6.91 ms (512-wide rows: 7.86 ms)
-//   FP32   256 wide, <= 4 per CU: 3.39 ms (256 wide, no cap: 3.47 ms)
-//   FP16   256 wide, no cap:      1.85 ms (512-wide rows: 2.06 ms)  Environment overrides (tuning): LBM_DENSE_BLOCK_X
+// whose DRAM efficiency drops as more streams are open per CU.  512^3 even
+// step, best of a sweep over widths 64-512 and 0-8 workgroups per CU:
+//   D3Q19 FP64   128 wide, <= 4 per CU: 6.91 ms (512-wide rows: 7.86 ms)
+//   D3Q19 FP32   256 wide, <= 4 per CU: 3.39 ms (256 wide, no cap: 3.47 ms)
+//   D3Q19 FP16   256 wide, no cap:      1.85 ms (512-wide rows: 2.06 ms)
+//   D3Q27 FP64   256 wide, <= 2 per CU: 9.31 ms (512-wide rows: 10.83 ms)
+//   D3Q27 FP32   128 wide, no cap:      4.69 ms (256 wide, no cap: 4.86 ms)
+//   D3Q27 FP16   128 wide, <= 8 per CU: 2.49 ms (512-wide rows: 2.70 ms)  Environment overrides (tuning): LBM_DENSE_BLOCK_X
 // and LBM_DENSE_BLOCKS_PER_CU for the unstaged kernels (the latter also for
 // the staged ones), LBM_DENSE_EVEN_BLOCK_X and LBM_DENSE_EVEN_BLOCKS_PER_CU for
 // the even step.
@@ -330,8 +333,14 @@ void set_launch_shapes(lbm_dense* h) {
   h->step_lds = lds_for_blocks_per_cu(env_int("LBM_DENSE_BLOCKS_PER_CU", 0));
   int even_bx = h->step_bx, even_k = 0;
   if (h->layout == LBM_LAYOUT_SOA) {
-    even_bx = h->storage == LBM_STORE_F64 ? 128 : 256;
-    even_k = h->storage == LBM_STORE_F16S ? 0 : 4;
+    static const int shape[2][3][2] = {
+        // {width, workgroups per CU} for F64, F32S, F16S
+        {{128, 4}, {256, 4}, {256, 0}},  // D3Q19
+        {{256, 2}, {128, 0}, {128, 8}},  // D3Q27
+    };
+    const int* sh = shape[h->q == 19 ? 0 : 1][h->storage];
+    even_bx = sh[0];
+    even_k = sh[1];
   }
   h->even_bx = clamp_bx(env_int("LBM_DENSE_EVEN_BLOCK_X", even_bx));
   h->even_lds = lds_for_blocks_per_cu(env_int("LBM_DENSE_EVEN_BLOCKS_PER_CU", even_k));
