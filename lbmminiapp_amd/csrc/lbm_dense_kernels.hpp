@@ -11,9 +11,11 @@
 // and of its MPI halo exchange, src/Communication.cpp:379-481).  Solid cells
 // and non-periodic faces use halfway bounce-back through a 32-bit link mask
 // per cell: bit s set <=> the neighbour at x + c_s is solid or outside a
-// non-periodic face; bit 31 marks a solid cell (skipped).  This is the same
-// rule as the reference adjacency (src/geometries/GeometryAAAB.cpp:67-85),
-// evaluated in-kernel instead of read from an N*Q int32 table.
+// non-periodic face; bit 31 marks a solid cell (no collision; its slots are
+// never read and, with DenseArgs::fill_solid, receive zeros so that every row
+// segment is stored whole).  This is the same rule as the reference adjacency
+// (src/geometries/GeometryAAAB.cpp:67-85), evaluated in-kernel instead of read
+// from an N*Q int32 table.
 //
 // Kernel kinds and the slots they touch (halfway bounce-back in brackets):
 //   AA_EVEN  read slot opp(s) at x, write slot s at x
