@@ -342,6 +342,11 @@ void set_launch_shapes(lbm_dense* h) {
     even_bx = sh[0];
     even_k = sh[1];
   }
+  int dev = 0, lds = 0;
+  if (hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
+      lds > 0)
+    h->lds_max = (size_t)lds;
   h->even_bx = clamp_bx(env_int("LBM_DENSE_EVEN_BLOCK_X", even_bx));
   h->even_lds = lds_for_blocks_per_cu(env_int("LBM_DENSE_EVEN_BLOCKS_PER_CU", even_k));
 }
@@ -375,8 +380,8 @@ DenseLaunchFn pick_staged(const lbm_dense* h, int kind) {
       h->nx > kDenseMaxBlock || (mode < 2 && (h->storage == LBM_STORE_F16S || mask)))
     return nullptr;
   const bool force = h->args.F0 != 0.0 || h->args.F1 != 0.0 || h->args.F2 != 0.0;
-  return h->q == 19 ? pick_staged19(h->storage, kind, force, h->nx, mask)
-                    : pick_staged27(h->storage, kind, force, h->nx, mask);
+  return h->q == 19 ? pick_staged19(h->storage, kind, force, h->nx, mask, h->lds_max)
+                    : pick_staged27(h->storage, kind, force, h->nx, mask, h->lds_max);
 }
 
 lbm_status launch_kind(lbm_dense* h, int kind, void* f0, void* f1) {

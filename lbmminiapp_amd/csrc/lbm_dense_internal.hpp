@@ -12,6 +12,7 @@ struct lbm_dense {
   // accesses are all row-aligned
   int step_bx = 256, even_bx = 256;
   size_t step_lds = 0, even_lds = 0;
+  size_t lds_max = 65536;  // the device's LDS limit per workgroup
   int64_t ncell = 0, nfluid = 0, t = 0;
   size_t esize = 8;
   void* f[2] = {nullptr, nullptr};

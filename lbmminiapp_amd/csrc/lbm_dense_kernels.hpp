@@ -414,19 +414,20 @@ DenseLaunchFn pick_staged_kind(int kind, bool force, bool mask) {
 
 // LDS-staged kernel for (storage, kind, mask), or nullptr when the staged form
 // does not apply (AA_EVEN needs no staging; with a mask only the pull gather
-// is staged) or its LDS would exceed 64 KB.
+// is staged) or its LDS would exceed lds_max, the device's limit per workgroup
+// (160 KiB on gfx950: a D3Q27 FP64 row of 512 needs 72 KiB).
 template <int Q>
-DenseLaunchFn pick_staged_q(int storage, int kind, bool force, int nx, bool mask) {
+DenseLaunchFn pick_staged_q(int storage, int kind, bool force, int nx, bool mask, size_t lds_max) {
   if (kind == K_AA_EVEN) return nullptr;
   switch (storage) {
     case LBM_STORE_F64:
-      if (staged_lds_bytes<Q, LBM_STORE_F64>(nx) > 65536) return nullptr;
+      if (staged_lds_bytes<Q, LBM_STORE_F64>(nx) > lds_max) return nullptr;
       return pick_staged_kind<Q, LBM_STORE_F64>(kind, force, mask);
     case LBM_STORE_F32S:
-      if (staged_lds_bytes<Q, LBM_STORE_F32S>(nx) > 65536) return nullptr;
+      if (staged_lds_bytes<Q, LBM_STORE_F32S>(nx) > lds_max) return nullptr;
       return pick_staged_kind<Q, LBM_STORE_F32S>(kind, force, mask);
     case LBM_STORE_F16S:
-      if (staged_lds_bytes<Q, LBM_STORE_F16S>(nx) > 65536) return nullptr;
+      if (staged_lds_bytes<Q, LBM_STORE_F16S>(nx) > lds_max) return nullptr;
       return pick_staged_kind<Q, LBM_STORE_F16S>(kind, force, mask);
   }
   return nullptr;
@@ -473,7 +474,7 @@ DenseLaunchFn pick_dense_q(int storage, int layout, int kind, bool mask, bool fo
 // force = false selects the zero-body-force kernels (only valid for F = 0).
 DenseLaunchFn pick_dense19(int storage, int layout, int kind, bool mask, bool force);
 DenseLaunchFn pick_dense27(int storage, int layout, int kind, bool mask, bool force);
-DenseLaunchFn pick_staged19(int storage, int kind, bool force, int nx, bool mask);
-DenseLaunchFn pick_staged27(int storage, int kind, bool force, int nx, bool mask);
+DenseLaunchFn pick_staged19(int storage, int kind, bool force, int nx, bool mask, size_t lds_max);
+DenseLaunchFn pick_staged27(int storage, int kind, bool force, int nx, bool mask, size_t lds_max);
 
 }  // namespace lbm

@@ -6,7 +6,8 @@ namespace lbm {
 DenseLaunchFn pick_dense27(int storage, int layout, int kind, bool mask, bool force) {
   return pick_dense_q<27>(storage, layout, kind, mask, force);
 }
-DenseLaunchFn pick_staged27(int storage, int kind, bool force, int nx, bool mask) {
-  return pick_staged_q<27>(storage, kind, force, nx, mask);
+DenseLaunchFn pick_staged27(int storage, int kind, bool force, int nx, bool mask,
+                            size_t lds_max) {
+  return pick_staged_q<27>(storage, kind, force, nx, mask, lds_max);
 }
 }  // namespace lbm

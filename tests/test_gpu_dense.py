@@ -211,6 +211,24 @@ def test_dense_staged_rows(q, pattern, storage, force, monkeypatch):
         assert np.abs(got - want).max() < tol
 
 
+@pytest.mark.parametrize("pattern", ["aa", "push", "pull"])
+def test_dense_staged_wide_rows_d3q27_f64(pattern):
+    """D3Q27 FP64 rows of 512 cells stage 18 x 512 doubles (72 KiB) per
+    workgroup, above 64 KiB (gfx950 allows 160 KiB): bit for bit with the
+    oracle."""
+    nx, ny, nz = 512, 4, 3
+    geom = O.DenseGeometry(nx, ny, nz)
+    fc = seeded_fc(geom, 27, seed=31)
+    omega, F = 1 / 0.55, (1e-6, 0.0, 2e-7)
+    print(f"[dense] wide staged rows q=27 f64 {pattern}", flush=True)
+    with DenseLattice(nx, ny, nz, q=27, pattern="aa" if pattern == "aa" else "ab",
+                      pull=pattern == "pull", storage="f64", omega=omega, force=F) as lat:
+        lat.set_state(fc)
+        lat.step(5)
+        got = lat.get_state()
+    assert np.array_equal(got, oracle_run(geom, 27, fc, pattern, 5, omega, F))
+
+
 def test_dense_config1_tgv_64cubed_1000_steps():
     """D3Q19 BGK, AA, SOA, FP64, 64^3 fully periodic, tau = 0.6, rho0 = 1,
     Taylor-Green u0 = 0.05 + seeded U(-1e-4, 1e-4) noise, 1000 steps, with the
