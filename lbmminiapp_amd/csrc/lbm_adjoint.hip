@@ -21,6 +21,7 @@
 // Memory: ceil(T/k) checkpoints + (k - 1) recomputed states of one segment +
 // two adjoint arrays; every forward step is computed twice.
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "lbm_dense_internal.hpp"
@@ -250,6 +251,8 @@ struct lbm_adjoint {
   size_t state_bytes = 0, lam_bytes = 0, extra_bytes = 0;
   std::vector<double> tau_host, gamma_host;
   double J = 0.0;
+  int step_bx = 256;        // transposed-step workgroup width
+  size_t step_lds = 0;      // its dynamic LDS floor (caps workgroups per CU)
 };
 
 namespace {
@@ -285,9 +288,9 @@ struct objdiff_u {
 template <int Q, int S>
 struct step_u {
   template <typename... A>
-  static void launch(dim3 g, dim3 b, hipStream_t st, bool mask, A... a) {
-    if (mask) hipLaunchKernelGGL((k_adj_step<Q, S, true>), g, b, 0, st, a...);
-    else hipLaunchKernelGGL((k_adj_step<Q, S, false>), g, b, 0, st, a...);
+  static void launch(dim3 g, dim3 b, hipStream_t st, size_t lds, bool mask, A... a) {
+    if (mask) hipLaunchKernelGGL((k_adj_step<Q, S, true>), g, b, lds, st, a...);
+    else hipLaunchKernelGGL((k_adj_step<Q, S, false>), g, b, lds, st, a...);
   }
 };
 
@@ -383,12 +386,12 @@ lbm_status backward_range(lbm_adjoint* h, const void* base, int64_t len, size_t 
       if (lbm_status st = launch_kind(d, K_AB_PUSH, const_cast<void*>(src), dst)) return st;
       src = dst;
     }
-    const dim3 g = adj_grid(d), b = adj_block(d);
+    const dim3 b(h->step_bx), g((d->nx + h->step_bx - 1) / h->step_bx, d->ny, d->nz);
     const bool mask = d->links != nullptr;
     for (int64_t j = len - 1; j >= 0; j--) {
       const void* state = j == 0 ? base : (char*)h->seg + (j - 1) * h->state_bytes;
-      launch_qs<step_u>(d, g, b, mask, state, (const void*)h->lam[cur], h->lam[1 - cur],
-                        h->grad[0], h->grad[1], d->args);
+      launch_qs<step_u>(d, g, b, h->step_lds, mask, state, (const void*)h->lam[cur],
+                        h->lam[1 - cur], h->grad[0], h->grad[1], d->args);
       LBM_LAUNCH_CHECK();
       cur = 1 - cur;
     }
@@ -478,6 +481,15 @@ lbm_status lbm_adjoint_create(const lbm_adjoint_config* cfg, lbm_adjoint** out) 
   h->every = cfg->checkpoint_every;
   if (k2 > 0) h->inner.push_back(k2);
   if (k3 > 0) h->inner.push_back(k3);
+  // transposed-step launch shape (LBM_ADJ_BLOCK_X, LBM_ADJ_BLOCKS_PER_CU: tuning)
+  {
+    const char* e = std::getenv("LBM_ADJ_BLOCK_X");
+    int bx = e ? std::atoi(e) : (int)adj_block(d).x;
+    bx = bx < 64 ? 64 : (bx > 256 ? 256 : bx / 64 * 64);
+    h->step_bx = bx;
+    const char* k = std::getenv("LBM_ADJ_BLOCKS_PER_CU");
+    h->step_lds = lds_for_blocks_per_cu(k ? std::atoi(k) : 0);
+  }
   h->state_bytes = (size_t)d->ncell * d->q * d->esize;
   h->lam_bytes = (size_t)d->ncell * d->q * (d->storage == LBM_STORE_F64 ? 8 : 4);
   if (cfg->tau) h->tau_host.assign(cfg->tau, cfg->tau + d->ncell);

@@ -35,6 +35,8 @@ dim3 block_for(int nx);
 dim3 grid_for(const lbm_dense* h);
 // one time-step launch of `kind` from f0 to f1 (f0 == f1 for AA), staged when possible
 lbm_status launch_kind(lbm_dense* h, int kind, void* f0, void* f1);
+// dynamic LDS that leaves room for at most k workgroups per CU (0: no cap)
+size_t lds_for_blocks_per_cu(int k);
 // device allocation accounted in h->device_bytes
 lbm_status dmalloc(lbm_dense* h, void** p, size_t bytes, const char* what);
 // lbm_dense_create; push_by_gather = false keeps AB push handles on the

@@ -1,6 +1,8 @@
 """Measured runs of the workloads beside the headline 512^3 D3Q19 AA line.
 Each prints one JSON line per run.
 
+  tgv64      D3Q19 AA SOA FP64 64^3 periodic Taylor-Green (u0 = 0.05 plus U(-1e-4, 1e-4)
+             noise, tau = 0.6), F = 0, 1000 steps.
   decks      The reference's benchmark decks: x-periodic cylinder pipe, simulationSize 2 / 4 / 8,
              D3Q19 SOA FP64 AA / AB push / AB pull, the deck's own step count.  Run through the
              drop-in driver (bin/lbm-proxy-app: reference fluid order and adjacency, sparse
@@ -20,7 +22,7 @@ algorithmic 2 Q b bytes per fluid node.
 
   porous_adjoint  the adjoint of the outlet flux with respect to tau on the porous workload.
 
-    python -m lbmminiapp_amd.workloads decks|channel|porous|porous_adjoint|precision [--steps K]
+    python -m lbmminiapp_amd.workloads tgv64|decks|channel|porous|porous_adjoint|precision [--steps K]
 """
 from __future__ import annotations
 
@@ -116,6 +118,18 @@ def _time_dense(lat, steps, warmup):
             "fluid_nodes": nf, "steps": steps, "mass_drift": (m1 - m0) / m0}
 
 
+def tgv64(args):
+    """Configuration 1: D3Q19 AA SOA FP64 64^3 periodic Taylor-Green (u0 = 0.05 plus seeded
+    U(-1e-4, 1e-4) noise, tau = 0.6), F = 0, 1000 steps."""
+    n = 64
+    steps = args.steps or 1000
+    with capi.DenseLattice(n, n, n, q=19, pattern="aa", storage="f64", omega=1 / 0.6) as lat:
+        lat.init_tgv(0.05, 1e-4, 1)
+        r = _time_dense(lat, steps, 10)
+    r.update({"workload": "D3Q19 AA SOA FP64 64^3 periodic Taylor-Green, tau = 0.6, F = 0"})
+    _emit(r)
+
+
 def channel(args):
     n = 256
     solid = capi.channel(n, n, n)
@@ -190,12 +204,12 @@ def precision(args):
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("workload", choices=["decks", "channel", "porous", "porous_adjoint",
-                                         "precision"])
+    ap.add_argument("workload", choices=["tgv64", "decks", "channel", "porous",
+                                         "porous_adjoint", "precision"])
     ap.add_argument("--steps", type=int, default=None, help="override the step count")
     args = ap.parse_args(argv)
-    {"decks": decks, "channel": channel, "porous": porous, "porous_adjoint": porous_adjoint,
-     "precision": precision}[args.workload](args)
+    {"tgv64": tgv64, "decks": decks, "channel": channel, "porous": porous,
+     "porous_adjoint": porous_adjoint, "precision": precision}[args.workload](args)
 
 
 if __name__ == "__main__":
