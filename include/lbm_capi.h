@@ -133,6 +133,11 @@ typedef struct {
   const double* tau;         /* host per-cell tau (extension), NULL = uniform omega */
   const double* gamma;       /* host per-cell Brinkman gamma in [0,1] (extension), NULL = none */
   double f16_scale;          /* FP16 deviation scale (power of two); 0 -> 4096 */
+  /* Periodic pressure drop along x (inlet x = 0, outlet x = nx - 1): a population
+   * crossing the periodic x boundary forward (x = nx - 1 -> 0) gains w_s * drho,
+   * one crossing backward loses it, so the inlet density exceeds the outlet's by
+   * about drho (pressure drop drho / 3).  0 = plain periodic; needs periodic[0]. */
+  double density_jump_x;
 } lbm_dense_config;
 
 typedef struct {
@@ -204,6 +209,7 @@ typedef struct {
   int32_t outlet_x;          /* outlet plane for objective 1 */
   int32_t checkpoint_every;  /* k1 >= 1 */
   int32_t checkpoint_inner[2]; /* {k2, k3}: 0 = level unused; k1 >= k2 >= k3 >= 1 where used */
+  double density_jump_x;     /* pressure-drop boundary of the forward model (lbm_dense_config) */
 } lbm_adjoint_config;
 
 lbm_status lbm_adjoint_create(const lbm_adjoint_config* cfg, lbm_adjoint** out);

@@ -59,7 +59,7 @@ class _DenseConfig(ctypes.Structure):
                 ("periodic", ctypes.c_int32 * 3), ("omega", ctypes.c_double),
                 ("force", ctypes.c_double * 3), ("solid", ctypes.c_void_p),
                 ("tau", ctypes.c_void_p), ("gamma", ctypes.c_void_p),
-                ("f16_scale", ctypes.c_double)]
+                ("f16_scale", ctypes.c_double), ("density_jump_x", ctypes.c_double)]
 
 
 class _AdjointConfig(ctypes.Structure):
@@ -69,7 +69,8 @@ class _AdjointConfig(ctypes.Structure):
                 ("force", ctypes.c_double * 3), ("solid", ctypes.c_void_p),
                 ("tau", ctypes.c_void_p), ("gamma", ctypes.c_void_p),
                 ("objective", ctypes.c_int32), ("outlet_x", ctypes.c_int32),
-                ("checkpoint_every", ctypes.c_int32), ("checkpoint_inner", ctypes.c_int32 * 2)]
+                ("checkpoint_every", ctypes.c_int32), ("checkpoint_inner", ctypes.c_int32 * 2),
+                ("density_jump_x", ctypes.c_double)]
 
 
 class _DenseInfo(ctypes.Structure):
@@ -270,7 +271,7 @@ class DenseLattice:
 
     def __init__(self, nx, ny, nz, q=19, pattern="aa", pull=False, layout="soa", storage="f64",
                  periodic=(1, 1, 1), omega=1.0, force=(0.0, 0.0, 0.0), solid=None, tau=None,
-                 gamma=None, f16_scale=0.0):
+                 gamma=None, f16_scale=0.0, density_jump_x=0.0):
         self.shape = (int(nx), int(ny), int(nz))
         self.q = q
         self.ncell = int(nx) * int(ny) * int(nz)
@@ -286,7 +287,7 @@ class DenseLattice:
                            STORAGE_NAMES[storage] if isinstance(storage, str) else storage,
                            (ctypes.c_int32 * 3)(*[int(p) for p in periodic]), float(omega),
                            (ctypes.c_double * 3)(*force), _p(self._solid), _p(self._tau),
-                           _p(self._gamma), float(f16_scale))
+                           _p(self._gamma), float(f16_scale), float(density_jump_x))
         self.h = ctypes.c_void_p()
         check(load().lbm_dense_create(ctypes.byref(cfg), ctypes.byref(self.h)))
 
@@ -364,7 +365,7 @@ class Adjoint:
     def __init__(self, nx, ny, nz, q=19, storage="f64", periodic=(1, 1, 1), omega=1.0,
                  force=(0.0, 0.0, 0.0), solid=None, tau=None, gamma=None,
                  objective="kinetic_energy", outlet_x=0, checkpoint_every=50,
-                 checkpoint_inner=(0, 0)):
+                 checkpoint_inner=(0, 0), density_jump_x=0.0):
         self.shape = (int(nx), int(ny), int(nz))
         self.q = q
         self.ncell = int(nx) * int(ny) * int(nz)
@@ -376,7 +377,8 @@ class Adjoint:
                              (ctypes.c_int32 * 3)(*[int(p) for p in periodic]), float(omega),
                              (ctypes.c_double * 3)(*force), _p(self._solid), _p(self._tau),
                              _p(self._gamma), obj, int(outlet_x), int(checkpoint_every),
-                             (ctypes.c_int32 * 2)(*[int(k) for k in checkpoint_inner]))
+                             (ctypes.c_int32 * 2)(*[int(k) for k in checkpoint_inner]),
+                             float(density_jump_x))
         self.h = ctypes.c_void_p()
         check(load().lbm_adjoint_create(ctypes.byref(cfg), ctypes.byref(self.h)))
 

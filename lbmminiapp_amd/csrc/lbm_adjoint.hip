@@ -467,6 +467,7 @@ lbm_status lbm_adjoint_create(const lbm_adjoint_config* cfg, lbm_adjoint** out) 
     dc.force[k] = cfg->force[k];
   }
   dc.omega = cfg->omega;
+  dc.density_jump_x = cfg->density_jump_x;  // additive on crossing links: the transpose is unchanged
   dc.solid = cfg->solid;
   dc.tau = cfg->tau;
   dc.gamma = cfg->gamma;

@@ -73,6 +73,15 @@ __global__ void __launch_bounds__(256)
 }
 
 // ---------------------------------------------------------------- state in / out
+// Sign of the pressure-drop jump between the stored value at canon_index and
+// the canonical f_s(x): non-zero only where a gathering convention reads
+// across the periodic x boundary (canonical = stored + sign * w_s * djump).
+template <int Q>
+__device__ __forceinline__ int canon_jump_sign(int mode, int s, int x, uint32_t lk, int nx) {
+  if (mode != C_GATHER || ((lk >> Lattice<Q>::opp(s)) & 1u)) return 0;
+  return gather_crossing(Lattice<Q>::c(s, 0), x, nx);
+}
+
 // f_s(x) -> storage, for every fluid cell; src is canonical [Q][ncell] (device).
 template <int Q, int STORE, int LAYOUT>
 __global__ void __launch_bounds__(256)
@@ -85,7 +94,8 @@ __global__ void __launch_bounds__(256)
   if (lk & SOLID_BIT) return;
   const Offsets off(x, y, z, a.nx, a.ny, a.nz);
   for (int s = 0; s < Q; s++) {
-    double v = src[a.ncell * s + cell];
+    double v = with_jump<Q>(src[a.ncell * s + cell], -canon_jump_sign<Q>(mode, s, x, lk, a.nx), s,
+                            a.djump);
     if constexpr (STORE != LBM_STORE_F64) v -= Lattice<Q>::w(s);
     st<STORE>(f, canon_index<Q, LAYOUT>(mode, s, cell, lk, off, a.ncell),
               (typename Storage<STORE>::C)v, a.scale);
@@ -110,9 +120,11 @@ __global__ void __launch_bounds__(256)
   for (int s = 0; s < Q; s++) {
     double v = 0.0;
     if (!(lk & SOLID_BIT))
-      v = to_f<STORE>(ld<STORE>(f, canon_index<Q, LAYOUT>(mode, s, cell, lk, off, a.ncell),
+      v = with_jump<Q>(
+          to_f<STORE>(ld<STORE>(f, canon_index<Q, LAYOUT>(mode, s, cell, lk, off, a.ncell),
                                 a.inv_scale),
-                      Lattice<Q>::w(s));
+                      Lattice<Q>::w(s)),
+          canon_jump_sign<Q>(mode, s, x, lk, a.nx), s, a.djump);
     dst[a.ncell * s + cell] = v;
   }
 }
@@ -132,10 +144,11 @@ __global__ void __launch_bounds__(256)
   if (!(lk & SOLID_BIT)) {
     const Offsets off(x, y, z, a.nx, a.ny, a.nz);
     for (int s = 0; s < Q; s++) {
-      const double v =
+      const double v = with_jump<Q>(
           to_f<STORE>(ld<STORE>(f, canon_index<Q, LAYOUT>(mode, s, cell, lk, off, a.ncell),
                                 a.inv_scale),
-                      Lattice<Q>::w(s));
+                      Lattice<Q>::w(s)),
+          canon_jump_sign<Q>(mode, s, x, lk, a.nx), s, a.djump);
       rho += v;
       m0 += v * Lattice<Q>::c(s, 0);
       m1 += v * Lattice<Q>::c(s, 1);
@@ -198,6 +211,7 @@ __global__ void __launch_bounds__(256)
     const double w = Lattice<Q>::w(s);
     double feq = w * rho * (1.0 + cdotv + 0.5 * cdotv * cdotv - 0.5 * vdotv);
     if constexpr (STORE != LBM_STORE_F64) feq = w * ((rho - 1.0) + rho * (cdotv + 0.5 * cdotv * cdotv - 0.5 * vdotv));
+    feq = with_jump<Q>(feq, -canon_jump_sign<Q>(mode, s, x, lk, a.nx), s, a.djump);
     st<STORE>(f, canon_index<Q, LAYOUT>(mode, s, cell, lk, off, a.ncell),
               (typename Storage<STORE>::C)feq, a.scale);
   }
@@ -469,6 +483,8 @@ lbm_status dense_create(const lbm_dense_config* cfg, lbm_dense** out, bool push_
               LBM_ERR_ARG, "lbm_dense_create: storage must be F64, F32S or F16S");
   LBM_REQUIRE(std::isfinite(cfg->omega) && cfg->omega > 0.0 && cfg->omega < 2.0, LBM_ERR_ARG,
               "lbm_dense_create: omega must be in (0, 2)");
+  LBM_REQUIRE(std::isfinite(cfg->density_jump_x) && (cfg->density_jump_x == 0.0 || cfg->periodic[0]),
+              LBM_ERR_ARG, "lbm_dense_create: density_jump_x needs a finite value and periodic x");
   lbm_dense* h = new lbm_dense;
   h->nx = cfg->nx;
   h->ny = cfg->ny;
@@ -601,6 +617,7 @@ lbm_status dense_create(const lbm_dense_config* cfg, lbm_dense** out, bool push_
   a.F2 = cfg->force[2];
   a.scale = (float)scale;
   a.inv_scale = (float)(1.0 / scale);
+  a.djump = cfg->density_jump_x;
   // rest state, as PropagationAA/AB::initialize (src/propagation/PropagationAA.cpp:60-123)
   if ((st = init_from(h, 0, nullptr, nullptr, 0.0, 0.0, 0))) return fail(st);
   if (h->pattern == LBM_PATTERN_AB &&

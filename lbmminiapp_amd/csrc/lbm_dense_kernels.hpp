@@ -50,7 +50,25 @@ struct DenseArgs {
   double omega;
   double F0, F1, F2;
   float scale, inv_scale;      // FP16 deviation scaling
+  double djump;                // density jump across the periodic x boundary (0: none)
 };
+
+// Pressure-drop boundary: the value a population carries across the periodic
+// x boundary changes by sign * w_s * djump (sign +1 forward, from x = nx - 1 to
+// 0; -1 backward).  Gathers: node x reads from x - c_s; scatters: node x writes
+// to x + c_s.
+__device__ __forceinline__ int gather_crossing(int cx, int x, int nx) {
+  return (cx > 0 && x == 0) ? 1 : ((cx < 0 && x == nx - 1) ? -1 : 0);
+}
+__device__ __forceinline__ int scatter_crossing(int cx, int x, int nx) {
+  return (cx > 0 && x == nx - 1) ? 1 : ((cx < 0 && x == 0) ? -1 : 0);
+}
+template <int Q, typename C>
+__device__ __forceinline__ C with_jump(C v, int sign, int s, double djump) {
+  if (sign == 0 || djump == 0.0) return v;
+  const C wj = (C)(Lattice<Q>::w(s) * djump);
+  return sign > 0 ? v + wj : v - wj;
+}
 
 template <int STORE>
 struct Storage;
@@ -181,17 +199,23 @@ __global__ void __launch_bounds__(kDenseMaxBlock)
       // the gathering kinds) share their boundary cache line with the
       // neighbouring wavefront and must stay cacheable.
       bool aligned = true;
+      int cross = 0;  // pressure-drop boundary crossed by this gather
       if constexpr (KIND == K_AA_EVEN) {
         idx = slot<Q, LAYOUT>(n, o, cell);
       } else if constexpr (KIND == K_AB_PUSH) {
         idx = slot<Q, LAYOUT>(n, s, cell);
       } else {
         aligned = cx == 0;
-        if (MASK && ((lk >> o) & 1u)) idx = slot<Q, LAYOUT>(n, o, cell);
-        else idx = slot<Q, LAYOUT>(n, s, cell + off.at(-cx, -cy, -cz));
+        if (MASK && ((lk >> o) & 1u)) {
+          idx = slot<Q, LAYOUT>(n, o, cell);
+        } else {
+          idx = slot<Q, LAYOUT>(n, s, cell + off.at(-cx, -cy, -cz));
+          cross = gather_crossing(cx, x, a.nx);
+        }
       }
       if (aligned && LAYOUT == LBM_LAYOUT_SOA) d[s] = ld_nt<STORE>(f0, idx, a.inv_scale);
       else d[s] = ld<STORE>(f0, idx, a.inv_scale);
+      d[s] = with_jump<Q>(d[s], cross, s, a.djump);
     }
 
     const double om = a.omega_field ? a.omega_field[cell] : a.omega;
@@ -221,7 +245,7 @@ __global__ void __launch_bounds__(kDenseMaxBlock)
       const int64_t sidx = slot<Q, LAYOUT>(n, KIND == K_AA_ODD ? o : s, cell + off.at(cx, cy, cz));
       const bool bb = MASK && ((lk >> s) & 1u);
       if (!bb) {
-        if (!solid) put(sidx, d[s], aligned);
+        if (!solid) put(sidx, with_jump<Q>(d[s], scatter_crossing(cx, x, a.nx), s, a.djump), aligned);
       } else {
         // halfway bounce-back slot: AA odd slot s at x, AB push slot opp(s) at x
         const int64_t bidx = slot<Q, LAYOUT>(n, KIND == K_AA_ODD ? s : o, cell);
@@ -302,7 +326,7 @@ __global__ void __launch_bounds__(kDenseMaxBlock)
       if (cx != 0) {
         int xs = x - cx;
         xs = xs < 0 ? xs + nx : (xs >= nx ? xs - nx : xs);
-        d[s] = lds[(k++) * nx + xs];
+        d[s] = with_jump<Q>(lds[(k++) * nx + xs], gather_crossing(cx, x, nx), s, a.djump);
       }
     }
   }
@@ -344,7 +368,7 @@ __global__ void __launch_bounds__(kDenseMaxBlock)
       } else {
         int xt = x + cx;
         xt = xt < 0 ? xt + nx : (xt >= nx ? xt - nx : xt);
-        lds[(k++) * nx + xt] = d[s];
+        lds[(k++) * nx + xt] = with_jump<Q>(d[s], scatter_crossing(cx, x, nx), s, a.djump);
       }
     }
     __syncthreads();

@@ -257,6 +257,114 @@ def run_ab(q, layout, geom_n, f, adj, steps, omega, pull=False, force=(1e-6, 0.0
     return src
 
 
+# --------------------------------------------------------------------------- pressure drop
+# Periodic pressure-drop boundary along x (an extension; the reference drives its
+# flow with a body force only).  A population streaming across the periodic x
+# boundary forward (from x = nx - 1 to x = 0) gains w_s * jump, one streaming
+# backward loses it.  The change is additive on the hop, and every storage slot
+# is the source of one gather and the target of one scatter, so it is applied
+# to the source slots before a gathering step and to the target slots after a
+# scattering step, around the C kernels above.
+def _slot_index(q, layout, n, s, i):
+    return n * s + i if layout == SOA else q * i + s
+
+
+def jump_link_signs(q, geom, layout, adj, hop):
+    """int8 [q, n]: +1 / -1 where the hop of direction s at node i crosses the
+    periodic x boundary forward / backward, 0 elsewhere and on bounce-back
+    links.  hop="gather": node i takes slot s from i - c_s (pull adjacency);
+    hop="scatter": node i sends direction s to i + c_s (push adjacency)."""
+    c, opp, _ = lattice(q)
+    n = geom.nfluid
+    x = geom.coords()[0]
+    adjm = from_storage(adj, q, n, layout)
+    i = np.arange(n)
+    sign = np.zeros((q, n), np.int8)
+    if not geom.periodic[0]:
+        return sign
+    for s in range(q):
+        cx = int(c[s, 0])
+        if hop == "gather":
+            g = np.where((cx > 0) & (x == 0), 1, np.where((cx < 0) & (x == geom.nx - 1), -1, 0))
+        else:
+            g = np.where((cx > 0) & (x == geom.nx - 1), 1, np.where((cx < 0) & (x == 0), -1, 0))
+        g[adjm[s] == _slot_index(q, layout, n, opp[s], i)] = 0
+        sign[s] = g
+    return sign
+
+
+def _add_jump(flat, q, targets, sign, jump):
+    """flat[targets[s, i]] += sign[s, i] * w_s * jump (targets distinct)."""
+    _, _, w = lattice(q)
+    for s in range(q):
+        wj = w[s] * jump
+        kp = targets[s][sign[s] > 0]
+        km = targets[s][sign[s] < 0]
+        flat[kp] = flat[kp] + wj
+        flat[km] = flat[km] - wj
+
+
+def run_ab_jump(q, layout, geom, f, adj, steps, omega, jump, pull=False,
+                force=(0.0, 0.0, 0.0)):
+    """AB push (adj = push adjacency) or pull (pull adjacency) with the
+    pressure-drop boundary; returns the latest storage array."""
+    n = geom.nfluid
+    sign = jump_link_signs(q, geom, layout, adj, "gather" if pull else "scatter")
+    targets = from_storage(adj, q, n, layout)
+    cur = f.copy()
+    for _ in range(steps):
+        if pull:
+            _add_jump(cur, q, targets, sign, jump)
+        cur = run_ab(q, layout, n, cur, adj, 1, omega, pull=pull, force=force)
+        if not pull:
+            _add_jump(cur, q, targets, sign, jump)
+    return cur
+
+
+def run_aa_jump(q, layout, geom, f, adj_pull, steps, omega, jump, force=(0.0, 0.0, 0.0), t0=0):
+    """AA in place with the pressure-drop boundary (odd steps gather from
+    x - c_s and scatter to x + c_s)."""
+    _, opp, _ = lattice(q)
+    n = geom.nfluid
+    adjm = from_storage(adj_pull, q, n, layout)
+    g_sign = jump_link_signs(q, geom, layout, adj_pull, "gather")
+    # direction s of node i is written to the pull-adjacency slot of opp(s)
+    s_targets = adjm[opp]
+    c, _, _ = lattice(q)
+    x = geom.coords()[0]
+    i = np.arange(n)
+    s_sign = np.zeros((q, n), np.int8)
+    if geom.periodic[0]:
+        for s in range(q):
+            cx = int(c[s, 0])
+            g = np.where((cx > 0) & (x == geom.nx - 1), 1, np.where((cx < 0) & (x == 0), -1, 0))
+            g[s_targets[s] == _slot_index(q, layout, n, s, i)] = 0  # bounce-back: slot s at i
+            s_sign[s] = g
+    for t in range(t0, t0 + steps):
+        if t % 2 == 1:
+            _add_jump(f, q, adjm, g_sign, jump)
+        run_aa(q, layout, n, f, adj_pull, 1, omega, force, t0=t)
+        if t % 2 == 1:
+            _add_jump(f, q, s_targets, s_sign, jump)
+    return f
+
+
+def gather_canonical_jump(flat, q, geom, layout, adj_pull, jump):
+    """Canonical f[q, n] of a gathering convention (AB pull storage, or AA after
+    an odd number of steps) with the pressure-drop boundary: the value gathered
+    across the x boundary carries the jump."""
+    n = geom.nfluid
+    adjm = from_storage(adj_pull, q, n, layout)
+    sign = jump_link_signs(q, geom, layout, adj_pull, "gather")
+    _, _, w = lattice(q)
+    out = np.empty((q, n))
+    for s in range(q):
+        v = np.asarray(flat)[adjm[s]].astype(np.float64)
+        wj = w[s] * jump
+        out[s] = np.where(sign[s] > 0, v + wj, np.where(sign[s] < 0, v - wj, v))
+    return out
+
+
 def run_unroll19(kind, n, f, adj, steps, omega, t0=0):
     """Unrolled Q19 SOA variants: kind in {"aa", "ab_push", "ab_pull"}."""
     L = lib()

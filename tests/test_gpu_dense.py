@@ -159,6 +159,87 @@ def test_dense_masked_staged_gather(q, periodic, storage, monkeypatch):
                 assert np.abs(got - want).max() < 1e-7, (pattern, fill)
 
 
+def oracle_run_jump(geom, q, fc, pattern, steps, omega, force, jump):
+    """Canonical state after `steps` steps with the pressure-drop boundary (oracle)."""
+    n = geom.nfluid
+    if pattern == "aa":
+        adj = geom.adjacency(q, O.SOA, O.ADJ_PULL)
+        f = O.to_storage(fc, q, O.SOA, aa_even_ready=True)
+        O.run_aa_jump(q, O.SOA, geom, f, adj, steps, omega, jump, force=force)
+        if steps % 2:
+            return O.gather_canonical_jump(f, q, geom, O.SOA, adj, jump)
+        return O.aa_canonical(f, q, n, O.SOA, adj, steps)
+    if pattern == "push":
+        adj = geom.adjacency(q, O.SOA, O.ADJ_PUSH)
+        out = O.run_ab_jump(q, O.SOA, geom, O.to_storage(fc, q, O.SOA), adj, steps, omega, jump,
+                            force=force)
+        return O.from_storage(out, q, n, O.SOA)
+    adj = geom.adjacency(q, O.SOA, O.ADJ_PULL)
+    adjm = O.from_storage(adj, q, n, O.SOA)
+    sign = O.jump_link_signs(q, geom, O.SOA, adj, "gather")
+    _, _, w = O.lattice(q)
+    flat = np.empty(q * n)
+    for s in range(q):  # storage whose jump-aware gather gives fc
+        wj = w[s] * jump
+        flat[adjm[s]] = np.where(sign[s] > 0, fc[s] - wj, np.where(sign[s] < 0, fc[s] + wj, fc[s]))
+    out = O.run_ab_jump(q, O.SOA, geom, flat, adj, steps, omega, jump, pull=True, force=force)
+    return O.gather_canonical_jump(out, q, geom, O.SOA, adj, jump)
+
+
+@pytest.mark.parametrize("q", [19, 27])
+@pytest.mark.parametrize("pattern", ["aa", "push", "pull"])
+@pytest.mark.parametrize("case", ["box128_staged", "spheres70", "spheres128_staged"])
+def test_dense_pressure_jump_vs_oracle(q, pattern, case, monkeypatch):
+    """Pressure-drop boundary (density jump across the periodic x boundary):
+    FP64 equals the oracle bit for bit after 7 steps, on a periodic box with
+    rows of 128 (LDS-staged kernels), on a channel with spheres (walls y,
+    unstaged kernels) and on the same channel with rows of 128 and staged
+    mode 2 (the masked staged gather), with a get/set round trip after 4 steps."""
+    nx = 70 if case == "spheres70" else 128
+    ny, nz = 8, 5
+    if case.startswith("box"):
+        solid, periodic = None, (1, 1, 1)
+    else:
+        m = np.zeros((nz, ny, nx), np.uint8)
+        m[:, 0, :] = 1
+        m[:, ny - 1, :] = 1
+        m = m.reshape(-1) | (O.uniform01(51, np.arange(nx * ny * nz, dtype=np.uint64)) < 0.2)
+        solid, periodic = m.astype(np.uint8), (1, 0, 1)
+    if case.endswith("staged"):
+        monkeypatch.setenv("LBM_DENSE_STAGED", "2")
+    geom = O.DenseGeometry(nx, ny, nz, solid, periodic=periodic)
+    fc = seeded_fc(geom, q, seed=37)
+    omega, force, jump = 1 / 0.7, (1e-6, 0.0, 0.0), 3e-3
+    want = oracle_run_jump(geom, q, fc, pattern, 7, omega, force, jump)
+    print(f"[dense] jump q={q} {pattern} {case}", flush=True)
+    with DenseLattice(nx, ny, nz, q=q, pattern="aa" if pattern == "aa" else "ab",
+                      pull=pattern == "pull", storage="f64", periodic=periodic, omega=omega,
+                      force=force, solid=solid, density_jump_x=jump) as lat:
+        lat.set_state(dense_canonical(geom, q, fc))
+        lat.step(4)
+        lat.set_state(lat.get_state())
+        lat.step(3)
+        got = lat.get_state()[:, geom.cell_of_fluid]
+    assert np.array_equal(got, want)
+
+
+def test_dense_pressure_jump_f32_and_mass():
+    """FP32 deviation storage with the pressure-drop boundary stays within 1e-7
+    of the FP64 oracle, and the total mass is unchanged to round-off."""
+    q, nx, ny, nz = 19, 128, 8, 5
+    geom = O.DenseGeometry(nx, ny, nz)
+    fc = seeded_fc(geom, q, seed=41)
+    omega, jump = 1 / 0.7, 3e-3
+    want = oracle_run_jump(geom, q, fc, "aa", 7, omega, (0.0, 0.0, 0.0), jump)
+    with DenseLattice(nx, ny, nz, q=q, pattern="aa", storage="f32s", omega=omega,
+                      density_jump_x=jump) as lat:
+        lat.set_state(fc)
+        lat.step(7)
+        got = lat.get_state()
+    assert np.abs(got - want).max() < 1e-7
+    assert abs(got.sum() - fc.sum()) < 1e-9 * fc.sum()
+
+
 @pytest.mark.parametrize("q", [19, 27])
 @pytest.mark.parametrize("pattern", ["aa", "push", "pull"])
 @pytest.mark.parametrize("storage", ["f64", "f32s"])
