@@ -12,8 +12,9 @@ Each prints one JSON line per run.
              x and z), body force 1e-6 along x, tau = 0.8, seeded spheres of radius 4-12 added
              until 5% of the cells are solid (seed 2), rest start, 5000 steps.
   porous     D3Q19 AB push, FP32 deviation storage, 384^3 packed spheres of radius 6-16 added
-             until 35% solid (seed 6), periodic, pressure drop 1e-4 along x applied as the
-             equivalent body force dp / nx, 3000 steps; SOA and AOS.
+             until 35% solid (seed 6), periodic, inlet/outlet pressure drop 1e-4 along x (the
+             pressure-drop boundary: density jump 3e-4 across the periodic x boundary), 3000
+             steps; SOA and AOS.
   precision  D3Q27 AA 512^3 Taylor-Green (u0 = 0.05, tau = 0.51), 2000 steps, FP64 vs FP32 / FP16
              deviation storage field by field (precision_study.py).
 
@@ -157,13 +158,19 @@ def porous(args):
     dp = 1e-4
     for layout in ("soa", "aos"):
         with capi.DenseLattice(n, n, n, q=19, pattern="ab", layout=layout, storage="f32s",
-                               periodic=(1, 1, 1), omega=1.0, force=(dp / n, 0, 0),
-                               solid=solid) as lat:
+                               periodic=(1, 1, 1), omega=1.0, solid=solid,
+                               density_jump_x=3.0 * dp) as lat:
             lat.init_tgv(0.0, 0.0, 1)
             r = _time_dense(lat, steps, 10)
+            rho, u = lat.moments()
+        fluid = solid == 0
+        x = np.arange(n ** 3) % n
         r.update({"workload": f"D3Q19 AB push {layout.upper()} FP32-deviation 384^3 porous, 35% "
-                              f"spheres r 6-16 (seed 6), dp = 1e-4 along x as body force dp/nx",
-                  "layout": layout, "solid_fraction": round(float((solid != 0).mean()), 4)})
+                              f"spheres r 6-16 (seed 6), inlet/outlet pressure drop 1e-4 along x",
+                  "layout": layout, "solid_fraction": round(float((solid != 0).mean()), 4),
+                  "mean_ux": float(u[fluid, 0].mean()),
+                  "rho_inlet_minus_outlet": float(rho[fluid & (x == 0)].mean()
+                                                  - rho[fluid & (x == n - 1)].mean())})
         _emit(r)
 
 
@@ -178,9 +185,9 @@ def porous_adjoint(args):
     tau = 0.55 + 0.45 * capi.smooth_random_field(n, n, n, 4.0, 7)
     steps = args.steps or 3000
     nf = int((solid == 0).sum())
-    with capi.Adjoint(n, n, n, q=19, storage="f32s", omega=1.0, force=(1e-4 / n, 0, 0),
-                      solid=solid, tau=tau, objective="outlet_flux", outlet_x=n - 1,
-                      checkpoint_every=250, checkpoint_inner=(25, 5)) as adj:
+    with capi.Adjoint(n, n, n, q=19, storage="f32s", omega=1.0, solid=solid, tau=tau,
+                      objective="outlet_flux", outlet_x=n - 1, checkpoint_every=250,
+                      checkpoint_inner=(25, 5), density_jump_x=3e-4) as adj:
         t0 = time.perf_counter()
         J = adj.forward(steps)
         fwd = time.perf_counter() - t0
@@ -188,7 +195,8 @@ def porous_adjoint(args):
         mem = adj.device_bytes()
     fluid = solid == 0
     _emit({"workload": "adjoint of outlet flux wrt tau, D3Q19 AB push FP32-deviation 384^3 porous "
-                       "(35% spheres r 6-16, seed 6), tau in [0.55, 1] (sigma 4, seed 7)",
+                       "(35% spheres r 6-16, seed 6), tau in [0.55, 1] (sigma 4, seed 7), inlet/"
+                       "outlet pressure drop 1e-4 along x",
            "steps": steps, "checkpoints": [250, 25, 5], "objective": J,
            "forward_s": round(fwd, 2), "backward_ms": round(ms, 1),
            "adjoint_mlups": round(nf * steps / (ms * 1e3), 1), "device_bytes": mem,
