@@ -58,6 +58,7 @@ void Parameters::readInput(std::string inputFile) {
       for (int d = 0; d < 3; d++) periodic_[d] = std::atoi(num(1 + d).c_str());
       periodicGiven_ = true;
     } else if (k == "solidfraction") solidFraction_ = std::atof(num(1).c_str());
+    else if (k == "densityjump") densityJump_ = std::atof(num(1).c_str());
     else if (k == "radiusmin") radiusMin_ = std::atof(num(1).c_str());
     else if (k == "radiusmax") radiusMax_ = std::atof(num(1).c_str());
     else if (k == "seed") seed_ = std::strtoull(num(1).c_str(), nullptr, 10);

@@ -12,6 +12,7 @@
 //   layout soa|aos        engine sparse|dense    storage f64|f32s|f16s (dense)
 //   geometry cylinder|box|channel|spheres|porous   size nx ny nz   periodic px py pz
 //   solidfraction f  radiusmin r  radiusmax r  seed s  init rest|tgv  u0 u  noise a
+//   densityjump d (engine dense: inlet/outlet pressure drop d / 3 on the periodic x axis)
 //   dump path            (binary dump of the final populations, for checks)
 #ifndef LBM_HOST_PARAMETERS_H
 #define LBM_HOST_PARAMETERS_H
@@ -52,6 +53,7 @@ class Parameters {
   int getPeriodic(int dim) const { return periodic_[dim]; }
   bool periodicGiven() const { return periodicGiven_; }
   double getSolidFraction() const { return solidFraction_; }
+  double getDensityJump() const { return densityJump_; }
   double getRadiusMin() const { return radiusMin_; }
   double getRadiusMax() const { return radiusMax_; }
   uint64_t getSeed() const { return seed_; }
@@ -74,6 +76,7 @@ class Parameters {
   int periodic_[3] = {1, 0, 0};
   bool periodicGiven_ = false;
   double solidFraction_ = 0.05, radiusMin_ = 4.0, radiusMax_ = 12.0;
+  double densityJump_ = 0.0;  // pressure-drop boundary of the dense engine (drho across x)
   uint64_t seed_ = 1;
   double u0_ = 0.05, noise_ = 0.0;
 };

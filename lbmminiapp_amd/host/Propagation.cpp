@@ -345,6 +345,7 @@ void PropagationDense::setup(void) {
     cfg.force[d] = parameters_.getForce()[d];
   }
   cfg.omega = parameters_.getOmega();
+  cfg.density_jump_x = parameters_.getDensityJump();
   bool any_solid = false;
   for (uint8_t v : geometry_.getSolidMask()) any_solid |= v != 0;
   cfg.solid = any_solid ? geometry_.getSolidMask().data() : nullptr;
