@@ -356,6 +356,14 @@ void set_launch_shapes(lbm_dense* h) {
     even_bx = sh[0];
     even_k = sh[1];
   }
+  // Non-temporal streaming pays when the populations exceed the 256 MiB
+  // memory-side cache; a grid that fits there runs cached (LBM_DENSE_NT=0|1
+  // overrides).
+  {
+    const size_t pop_bytes =
+        (h->pattern == LBM_PATTERN_AB ? 2 : 1) * (size_t)h->ncell * h->q * h->esize;
+    h->nt = env_int("LBM_DENSE_NT", pop_bytes > ((size_t)256 << 20) ? 1 : 0) != 0;
+  }
   int dev = 0, lds = 0;
   if (hipGetDevice(&dev) == hipSuccess &&
       hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess &&
@@ -374,8 +382,8 @@ int canon_mode(const lbm_dense* h) {
 DenseLaunchFn pick_step(const lbm_dense* h, int kind) {
   const bool mask = h->links != nullptr;
   const bool force = h->args.F0 != 0.0 || h->args.F1 != 0.0 || h->args.F2 != 0.0;
-  return h->q == 19 ? pick_dense19(h->storage, h->layout, kind, mask, force)
-                    : pick_dense27(h->storage, h->layout, kind, mask, force);
+  return h->q == 19 ? pick_dense19(h->storage, h->layout, kind, mask, force, h->nt)
+                    : pick_dense27(h->storage, h->layout, kind, mask, force, h->nt);
 }
 
 // The LDS-staged kernels cover SOA grids whose rows fit one workgroup (nx a
@@ -394,8 +402,8 @@ DenseLaunchFn pick_staged(const lbm_dense* h, int kind) {
       h->nx > kDenseMaxBlock || (mode < 2 && (h->storage == LBM_STORE_F16S || mask)))
     return nullptr;
   const bool force = h->args.F0 != 0.0 || h->args.F1 != 0.0 || h->args.F2 != 0.0;
-  return h->q == 19 ? pick_staged19(h->storage, kind, force, h->nx, mask, h->lds_max)
-                    : pick_staged27(h->storage, kind, force, h->nx, mask, h->lds_max);
+  return h->q == 19 ? pick_staged19(h->storage, kind, force, h->nx, mask, h->lds_max, h->nt)
+                    : pick_staged27(h->storage, kind, force, h->nx, mask, h->lds_max, h->nt);
 }
 
 lbm_status launch_kind(lbm_dense* h, int kind, void* f0, void* f1) {
@@ -496,9 +504,9 @@ lbm_status dense_create(const lbm_dense_config* cfg, lbm_dense** out, bool push_
   h->storage = cfg->storage;
   for (int k = 0; k < 3; k++) h->periodic[k] = cfg->periodic[k] ? 1 : 0;
   if (const char* e = std::getenv("LBM_DENSE_STAGED")) h->staged_mode = std::atoi(e);
-  set_launch_shapes(h);
   h->ncell = (int64_t)cfg->nx * cfg->ny * cfg->nz;
   h->esize = cfg->storage == LBM_STORE_F64 ? 8 : (cfg->storage == LBM_STORE_F32S ? 4 : 2);
+  set_launch_shapes(h);  // after ncell and esize: the cache policy depends on the footprint
   // fluid count (host mask)
   h->nfluid = h->ncell;
   if (cfg->solid) {
@@ -512,13 +520,18 @@ lbm_status dense_create(const lbm_dense_config* cfg, lbm_dense** out, bool push_
   // step out of phase: with P_0 = S^-1 f_0, (C S)^T P_0 = S^-1 (S C)^T f_0, so
   // the push state f_T is the gather of the pull storage, which is how every
   // readout of a pull handle reads it.  Same per-node arithmetic, bit-identical
-  // states; the masked scatter measured 14.7k vs 20.4k MLUPS for the gather
-  // (D3Q19 FP32 deviations, 384^3, 35% solid).  LBM_DENSE_PUSH_GATHER=0 keeps
-  // the scatter kernel (A/B timing).
+  // states.  The gather measured faster when the two arrays stream from HBM
+  // (384^3 porous, 8.6 GB: 20.4k vs 14.7k MLUPS; reference cylinder at
+  // simulationSize 8, 3.5 GB: 18.1k vs 14.9k) and slower when they nearly fit
+  // the 256 MiB memory-side cache (simulationSize 4, 460 MB: 14.6k vs 17.9k),
+  // so it applies above 1 GiB of populations.  LBM_DENSE_PUSH_GATHER=0|1
+  // overrides (A/B timing).
   if (push_by_gather && h->pattern == LBM_PATTERN_AB && !h->pull && need_links &&
       h->layout == LBM_LAYOUT_SOA) {
+    const size_t pop_bytes = 2 * (size_t)h->ncell * h->q * h->esize;
     const char* e = std::getenv("LBM_DENSE_PUSH_GATHER");
-    if (!(e && e[0] == '0')) h->pull = 1;
+    const bool gather = e ? e[0] != '0' : pop_bytes > ((size_t)1 << 30);
+    if (gather) h->pull = 1;
   }
   lbm_status st = LBM_OK;
   auto fail = [&](lbm_status s) {

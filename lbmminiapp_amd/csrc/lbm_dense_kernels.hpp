@@ -119,6 +119,21 @@ __device__ __forceinline__ void st_nt(typename Storage<STORE>::T* p, int64_t idx
   else __builtin_nontemporal_store(v, p + idx);
 }
 
+// Streaming accesses of the time-step kernels: non-temporal when the grid
+// exceeds the memory-side cache (NT), cached when it fits there.
+template <int STORE, bool NT>
+__device__ __forceinline__ typename Storage<STORE>::C ld_s(const typename Storage<STORE>::T* p,
+                                                           int64_t idx, float inv_scale) {
+  if constexpr (NT) return ld_nt<STORE>(p, idx, inv_scale);
+  else return ld<STORE>(p, idx, inv_scale);
+}
+template <int STORE, bool NT>
+__device__ __forceinline__ void st_s(typename Storage<STORE>::T* p, int64_t idx,
+                                     typename Storage<STORE>::C v, float scale) {
+  if constexpr (NT) st_nt<STORE>(p, idx, v, scale);
+  else st<STORE>(p, idx, v, scale);
+}
+
 template <int Q, int LAYOUT>
 __device__ __forceinline__ int64_t slot(int64_t ncell, int s, int64_t cell) {
   return LAYOUT == LBM_LAYOUT_SOA ? ncell * s + cell : cell * Q + s;
@@ -170,7 +185,7 @@ constexpr int kDenseMaxBlock = 512;
 //                      the slot at T that streaming would have written
 //     T outside        fluid x bounces back; solid x stores a zero into that
 //                      bounce-back slot
-template <int Q, int STORE, int LAYOUT, int KIND, bool MASK, bool FORCE>
+template <int Q, int STORE, int LAYOUT, int KIND, bool MASK, bool FORCE, bool NT = true>
 __global__ void __launch_bounds__(kDenseMaxBlock)
     k_dense(const typename Storage<STORE>::T* f0, typename Storage<STORE>::T* f1, DenseArgs a) {
   using C = typename Storage<STORE>::C;
@@ -183,7 +198,11 @@ __global__ void __launch_bounds__(kDenseMaxBlock)
   if constexpr (MASK) {
     lk = a.links[cell];
     solid = (lk & SOLID_BIT) != 0;
-    if (solid && !a.fill_solid) return;
+    // a wavefront of solid cells only completes no partly written line: its
+    // zeros would be pure extra traffic (whole solid regions of a box).  The
+    // vote is taken by every lane of the wavefront, before any lane leaves.
+    const bool all_solid = __all(solid);
+    if (solid && (!a.fill_solid || all_solid)) return;
   }
   const Offsets off(x, y, z, a.nx, a.ny, a.nz);
   const int64_t n = a.ncell;
@@ -213,7 +232,7 @@ __global__ void __launch_bounds__(kDenseMaxBlock)
           cross = gather_crossing(cx, x, a.nx);
         }
       }
-      if (aligned && LAYOUT == LBM_LAYOUT_SOA) d[s] = ld_nt<STORE>(f0, idx, a.inv_scale);
+      if (aligned && LAYOUT == LBM_LAYOUT_SOA) d[s] = ld_s<STORE, NT>(f0, idx, a.inv_scale);
       else d[s] = ld<STORE>(f0, idx, a.inv_scale);
       d[s] = with_jump<Q>(d[s], cross, s, a.djump);
     }
@@ -230,7 +249,7 @@ __global__ void __launch_bounds__(kDenseMaxBlock)
   }
 
   auto put = [&](int64_t idx, C v, bool aligned) {
-    if (aligned && LAYOUT == LBM_LAYOUT_SOA) st_nt<STORE>(f1, idx, v, a.scale);
+    if (aligned && LAYOUT == LBM_LAYOUT_SOA) st_s<STORE, NT>(f1, idx, v, a.scale);
     else st<STORE>(f1, idx, v, a.scale);
   };
 #pragma unroll
@@ -282,7 +301,7 @@ __host__ __device__ constexpr int n_xshift() {
 // whose neighbour x - c_s is solid or outside then takes slot opp(s) at x
 // instead, solid nodes skip the collision and store zeros (a.fill_solid), and
 // all threads stay for the barrier.
-template <int Q, int STORE, int KIND, bool FORCE, bool MASK = false>
+template <int Q, int STORE, int KIND, bool FORCE, bool MASK = false, bool NT = true>
 __global__ void __launch_bounds__(kDenseMaxBlock)
     k_dense_staged(const typename Storage<STORE>::T* f0, typename Storage<STORE>::T* f1,
                    DenseArgs a) {
@@ -308,12 +327,12 @@ __global__ void __launch_bounds__(kDenseMaxBlock)
       const int cx = Lattice<Q>::c(s, 0), cy = Lattice<Q>::c(s, 1), cz = Lattice<Q>::c(s, 2);
       if constexpr (GATHER) {
         // element x of slot s in row (y - cy, z - cz)
-        const C v = ld_nt<STORE>(f0, slot<Q, LBM_LAYOUT_SOA>(n, s, cell + off.at(0, -cy, -cz)),
+        const C v = ld_s<STORE, NT>(f0, slot<Q, LBM_LAYOUT_SOA>(n, s, cell + off.at(0, -cy, -cz)),
                                  a.inv_scale);
         if (cx == 0) d[s] = v;
         else lds[(k++) * nx + x] = v;
       } else {
-        d[s] = ld_nt<STORE>(f0, slot<Q, LBM_LAYOUT_SOA>(n, s, cell), a.inv_scale);
+        d[s] = ld_s<STORE, NT>(f0, slot<Q, LBM_LAYOUT_SOA>(n, s, cell), a.inv_scale);
       }
     }
   }
@@ -339,7 +358,7 @@ __global__ void __launch_bounds__(kDenseMaxBlock)
 #pragma unroll
     for (int s = 0; s < Q; s++) {
       const int o = Lattice<Q>::opp(s);
-      if (!solid && ((lk >> o) & 1u)) d[s] = ld_nt<STORE>(f0, slot<Q, LBM_LAYOUT_SOA>(n, o, cell), a.inv_scale);
+      if (!solid && ((lk >> o) & 1u)) d[s] = ld_s<STORE, NT>(f0, slot<Q, LBM_LAYOUT_SOA>(n, o, cell), a.inv_scale);
     }
   }
 
@@ -364,7 +383,7 @@ __global__ void __launch_bounds__(kDenseMaxBlock)
       const int cx = Lattice<Q>::c(s, 0), cy = Lattice<Q>::c(s, 1), cz = Lattice<Q>::c(s, 2);
       const int tgt = KIND == K_AA_ODD ? o : s;
       if (cx == 0) {
-        st_nt<STORE>(f1, slot<Q, LBM_LAYOUT_SOA>(n, tgt, cell + off.at(0, cy, cz)), d[s], a.scale);
+        st_s<STORE, NT>(f1, slot<Q, LBM_LAYOUT_SOA>(n, tgt, cell + off.at(0, cy, cz)), d[s], a.scale);
       } else {
         int xt = x + cx;
         xt = xt < 0 ? xt + nx : (xt >= nx ? xt - nx : xt);
@@ -379,14 +398,14 @@ __global__ void __launch_bounds__(kDenseMaxBlock)
       const int cx = Lattice<Q>::c(s, 0), cy = Lattice<Q>::c(s, 1), cz = Lattice<Q>::c(s, 2);
       const int tgt = KIND == K_AA_ODD ? o : s;
       if (cx != 0)
-        st_nt<STORE>(f1, slot<Q, LBM_LAYOUT_SOA>(n, tgt, cell + off.at(0, cy, cz)),
+        st_s<STORE, NT>(f1, slot<Q, LBM_LAYOUT_SOA>(n, tgt, cell + off.at(0, cy, cz)),
                      lds[(k++) * nx + x], a.scale);
     }
   } else {
     if (solid && !a.fill_solid) return;
 #pragma unroll
     for (int s = 0; s < Q; s++)
-      st_nt<STORE>(f1, slot<Q, LBM_LAYOUT_SOA>(n, s, cell), solid ? C(0) : d[s], a.scale);
+      st_s<STORE, NT>(f1, slot<Q, LBM_LAYOUT_SOA>(n, s, cell), solid ? C(0) : d[s], a.scale);
   }
 }
 
@@ -401,37 +420,38 @@ constexpr size_t staged_lds_bytes(int nx) {
 using DenseLaunchFn = void (*)(dim3 grid, dim3 block, hipStream_t stream, void* f0, void* f1,
                                const DenseArgs& a, size_t lds_min);
 
-template <int Q, int STORE, int LAYOUT, int KIND, bool MASK, bool FORCE>
+template <int Q, int STORE, int LAYOUT, int KIND, bool MASK, bool FORCE, bool NT>
 void launch_dense(dim3 grid, dim3 block, hipStream_t stream, void* f0, void* f1,
                   const DenseArgs& a, size_t lds_min) {
   using T = typename Storage<STORE>::T;
-  hipLaunchKernelGGL((k_dense<Q, STORE, LAYOUT, KIND, MASK, FORCE>), grid, block, lds_min, stream,
+  hipLaunchKernelGGL((k_dense<Q, STORE, LAYOUT, KIND, MASK, FORCE, NT>), grid, block, lds_min, stream,
                      (const T*)f0, (T*)f1, a);
 }
 
-template <int Q, int STORE, int KIND, bool FORCE, bool MASK>
+template <int Q, int STORE, int KIND, bool FORCE, bool MASK, bool NT>
 void launch_staged(dim3 grid, dim3 block, hipStream_t stream, void* f0, void* f1,
                    const DenseArgs& a, size_t lds_min) {
   using T = typename Storage<STORE>::T;
   size_t lds = staged_lds_bytes<Q, STORE>(a.nx);
   if (lds < lds_min) lds = lds_min;
-  hipLaunchKernelGGL((k_dense_staged<Q, STORE, KIND, FORCE, MASK>), grid, block, lds, stream,
+  hipLaunchKernelGGL((k_dense_staged<Q, STORE, KIND, FORCE, MASK, NT>), grid, block, lds, stream,
                      (const T*)f0, (T*)f1, a);
 }
 
-template <int Q, int STORE, int KIND, bool MASK>
+template <int Q, int STORE, int KIND, bool MASK, bool NT>
 DenseLaunchFn pick_staged_force(bool force) {
-  return force ? launch_staged<Q, STORE, KIND, true, MASK>
-               : launch_staged<Q, STORE, KIND, false, MASK>;
+  return force ? launch_staged<Q, STORE, KIND, true, MASK, NT>
+               : launch_staged<Q, STORE, KIND, false, MASK, NT>;
 }
 
-template <int Q, int STORE>
+template <int Q, int STORE, bool NT>
 DenseLaunchFn pick_staged_kind(int kind, bool force, bool mask) {
-  if (mask) return kind == K_AB_PULL ? pick_staged_force<Q, STORE, K_AB_PULL, true>(force) : nullptr;
+  if (mask)
+    return kind == K_AB_PULL ? pick_staged_force<Q, STORE, K_AB_PULL, true, NT>(force) : nullptr;
   switch (kind) {
-    case K_AA_ODD: return pick_staged_force<Q, STORE, K_AA_ODD, false>(force);
-    case K_AB_PUSH: return pick_staged_force<Q, STORE, K_AB_PUSH, false>(force);
-    case K_AB_PULL: return pick_staged_force<Q, STORE, K_AB_PULL, false>(force);
+    case K_AA_ODD: return pick_staged_force<Q, STORE, K_AA_ODD, false, NT>(force);
+    case K_AB_PUSH: return pick_staged_force<Q, STORE, K_AB_PUSH, false, NT>(force);
+    case K_AB_PULL: return pick_staged_force<Q, STORE, K_AB_PULL, false, NT>(force);
   }
   return nullptr;
 }
@@ -440,65 +460,80 @@ DenseLaunchFn pick_staged_kind(int kind, bool force, bool mask) {
 // does not apply (AA_EVEN needs no staging; with a mask only the pull gather
 // is staged) or its LDS would exceed lds_max, the device's limit per workgroup
 // (160 KiB on gfx950: a D3Q27 FP64 row of 512 needs 72 KiB).
-template <int Q>
-DenseLaunchFn pick_staged_q(int storage, int kind, bool force, int nx, bool mask, size_t lds_max) {
+template <int Q, bool NT>
+DenseLaunchFn pick_staged_qn(int storage, int kind, bool force, int nx, bool mask, size_t lds_max) {
   if (kind == K_AA_EVEN) return nullptr;
   switch (storage) {
     case LBM_STORE_F64:
       if (staged_lds_bytes<Q, LBM_STORE_F64>(nx) > lds_max) return nullptr;
-      return pick_staged_kind<Q, LBM_STORE_F64>(kind, force, mask);
+      return pick_staged_kind<Q, LBM_STORE_F64, NT>(kind, force, mask);
     case LBM_STORE_F32S:
       if (staged_lds_bytes<Q, LBM_STORE_F32S>(nx) > lds_max) return nullptr;
-      return pick_staged_kind<Q, LBM_STORE_F32S>(kind, force, mask);
+      return pick_staged_kind<Q, LBM_STORE_F32S, NT>(kind, force, mask);
     case LBM_STORE_F16S:
       if (staged_lds_bytes<Q, LBM_STORE_F16S>(nx) > lds_max) return nullptr;
-      return pick_staged_kind<Q, LBM_STORE_F16S>(kind, force, mask);
-  }
-  return nullptr;
-}
-
-template <int Q, int STORE, int LAYOUT, int KIND>
-DenseLaunchFn pick_flags(bool mask, bool force) {
-  if (mask) return force ? launch_dense<Q, STORE, LAYOUT, KIND, true, true>
-                         : launch_dense<Q, STORE, LAYOUT, KIND, true, false>;
-  return force ? launch_dense<Q, STORE, LAYOUT, KIND, false, true>
-               : launch_dense<Q, STORE, LAYOUT, KIND, false, false>;
-}
-
-template <int Q, int STORE, int LAYOUT>
-DenseLaunchFn pick_kind(int kind, bool mask, bool force) {
-  switch (kind) {
-    case K_AA_EVEN: return pick_flags<Q, STORE, LAYOUT, K_AA_EVEN>(mask, force);
-    case K_AA_ODD: return pick_flags<Q, STORE, LAYOUT, K_AA_ODD>(mask, force);
-    case K_AB_PUSH: return pick_flags<Q, STORE, LAYOUT, K_AB_PUSH>(mask, force);
-    case K_AB_PULL: return pick_flags<Q, STORE, LAYOUT, K_AB_PULL>(mask, force);
+      return pick_staged_kind<Q, LBM_STORE_F16S, NT>(kind, force, mask);
   }
   return nullptr;
 }
 
 template <int Q>
-DenseLaunchFn pick_dense_q(int storage, int layout, int kind, bool mask, bool force) {
+DenseLaunchFn pick_staged_q(int storage, int kind, bool force, int nx, bool mask, size_t lds_max,
+                            bool nt) {
+  return nt ? pick_staged_qn<Q, true>(storage, kind, force, nx, mask, lds_max)
+            : pick_staged_qn<Q, false>(storage, kind, force, nx, mask, lds_max);
+}
+
+template <int Q, int STORE, int LAYOUT, int KIND, bool NT>
+DenseLaunchFn pick_flags(bool mask, bool force) {
+  if (mask) return force ? launch_dense<Q, STORE, LAYOUT, KIND, true, true, NT>
+                         : launch_dense<Q, STORE, LAYOUT, KIND, true, false, NT>;
+  return force ? launch_dense<Q, STORE, LAYOUT, KIND, false, true, NT>
+               : launch_dense<Q, STORE, LAYOUT, KIND, false, false, NT>;
+}
+
+template <int Q, int STORE, int LAYOUT, bool NT>
+DenseLaunchFn pick_kind(int kind, bool mask, bool force) {
+  switch (kind) {
+    case K_AA_EVEN: return pick_flags<Q, STORE, LAYOUT, K_AA_EVEN, NT>(mask, force);
+    case K_AA_ODD: return pick_flags<Q, STORE, LAYOUT, K_AA_ODD, NT>(mask, force);
+    case K_AB_PUSH: return pick_flags<Q, STORE, LAYOUT, K_AB_PUSH, NT>(mask, force);
+    case K_AB_PULL: return pick_flags<Q, STORE, LAYOUT, K_AB_PULL, NT>(mask, force);
+  }
+  return nullptr;
+}
+
+template <int Q, bool NT>
+DenseLaunchFn pick_dense_qn(int storage, int layout, int kind, bool mask, bool force) {
   if (layout == LBM_LAYOUT_SOA) {
     switch (storage) {
-      case LBM_STORE_F64: return pick_kind<Q, LBM_STORE_F64, LBM_LAYOUT_SOA>(kind, mask, force);
-      case LBM_STORE_F32S: return pick_kind<Q, LBM_STORE_F32S, LBM_LAYOUT_SOA>(kind, mask, force);
-      case LBM_STORE_F16S: return pick_kind<Q, LBM_STORE_F16S, LBM_LAYOUT_SOA>(kind, mask, force);
+      case LBM_STORE_F64: return pick_kind<Q, LBM_STORE_F64, LBM_LAYOUT_SOA, NT>(kind, mask, force);
+      case LBM_STORE_F32S: return pick_kind<Q, LBM_STORE_F32S, LBM_LAYOUT_SOA, NT>(kind, mask, force);
+      case LBM_STORE_F16S: return pick_kind<Q, LBM_STORE_F16S, LBM_LAYOUT_SOA, NT>(kind, mask, force);
     }
   } else {
     switch (storage) {
-      case LBM_STORE_F64: return pick_kind<Q, LBM_STORE_F64, LBM_LAYOUT_AOS>(kind, mask, force);
-      case LBM_STORE_F32S: return pick_kind<Q, LBM_STORE_F32S, LBM_LAYOUT_AOS>(kind, mask, force);
-      case LBM_STORE_F16S: return pick_kind<Q, LBM_STORE_F16S, LBM_LAYOUT_AOS>(kind, mask, force);
+      case LBM_STORE_F64: return pick_kind<Q, LBM_STORE_F64, LBM_LAYOUT_AOS, NT>(kind, mask, force);
+      case LBM_STORE_F32S: return pick_kind<Q, LBM_STORE_F32S, LBM_LAYOUT_AOS, NT>(kind, mask, force);
+      case LBM_STORE_F16S: return pick_kind<Q, LBM_STORE_F16S, LBM_LAYOUT_AOS, NT>(kind, mask, force);
     }
   }
   return nullptr;
 }
 
+template <int Q>
+DenseLaunchFn pick_dense_q(int storage, int layout, int kind, bool mask, bool force, bool nt) {
+  return nt ? pick_dense_qn<Q, true>(storage, layout, kind, mask, force)
+            : pick_dense_qn<Q, false>(storage, layout, kind, mask, force);
+}
+
 // Defined in lbm_dense_q19.hip and lbm_dense_q27.hip (one translation unit per Q).
 // force = false selects the zero-body-force kernels (only valid for F = 0).
-DenseLaunchFn pick_dense19(int storage, int layout, int kind, bool mask, bool force);
-DenseLaunchFn pick_dense27(int storage, int layout, int kind, bool mask, bool force);
-DenseLaunchFn pick_staged19(int storage, int kind, bool force, int nx, bool mask, size_t lds_max);
-DenseLaunchFn pick_staged27(int storage, int kind, bool force, int nx, bool mask, size_t lds_max);
+DenseLaunchFn pick_dense19(int storage, int layout, int kind, bool mask, bool force, bool nt);
+DenseLaunchFn pick_dense27(int storage, int layout, int kind, bool mask, bool force, bool nt);
+DenseLaunchFn pick_staged19(int storage, int kind, bool force, int nx, bool mask, size_t lds_max,
+                            bool nt);
+DenseLaunchFn pick_staged27(int storage, int kind, bool force, int nx, bool mask, size_t lds_max,
+                            bool nt);
 
 }  // namespace lbm

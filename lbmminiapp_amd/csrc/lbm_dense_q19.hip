@@ -3,11 +3,11 @@
 #include "lbm_dense_kernels.hpp"
 
 namespace lbm {
-DenseLaunchFn pick_dense19(int storage, int layout, int kind, bool mask, bool force) {
-  return pick_dense_q<19>(storage, layout, kind, mask, force);
+DenseLaunchFn pick_dense19(int storage, int layout, int kind, bool mask, bool force, bool nt) {
+  return pick_dense_q<19>(storage, layout, kind, mask, force, nt);
 }
 DenseLaunchFn pick_staged19(int storage, int kind, bool force, int nx, bool mask,
-                            size_t lds_max) {
-  return pick_staged_q<19>(storage, kind, force, nx, mask, lds_max);
+                            size_t lds_max, bool nt) {
+  return pick_staged_q<19>(storage, kind, force, nx, mask, lds_max, nt);
 }
 }  // namespace lbm

@@ -12,6 +12,8 @@
 // Launches go to the null stream so that they are ordered with the caller's
 // blocking hipMemcpy traffic, as the reference's kernel calls are ordered with
 // its host code.
+#include <cstdlib>
+
 #include "lbm_collide.hpp"
 #include "lbm_internal.hpp"
 
@@ -22,8 +24,25 @@ __device__ __forceinline__ int64_t pidx(int64_t n, int s, int64_t i) {
   return LAYOUT == LBM_LAYOUT_SOA ? n * s + i : (int64_t)Q * i + s;
 }
 
+// Node-aligned accesses (a work-item's own slots and adjacency entries) are
+// touched once per step.  With NT they stream past the caches, which pays when
+// the arrays exceed the memory-side cache; arrays that fit in it are faster
+// cached (the reference's simulationSize 4 deck, 175 MB: 15.2k vs 13.6k MLUPS
+// AA; simulationSize 8, 1.35 GB: 13.7k vs 14.8k).  Accesses through the
+// adjacency stay cacheable, since neighbouring work-items share their lines.
+template <bool NT, typename T>
+__device__ __forceinline__ T ldn(const T* p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+template <bool NT, typename T>
+__device__ __forceinline__ void stn(T* p, T v) {
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
 // AA even: src/kernels/KernelAA_SOA_Base.cpp:173-254 (AOS: KernelAA_AOS_Base.cpp:174-254)
-template <int Q, typename T, int LAYOUT>
+template <int Q, typename T, int LAYOUT, bool NT>
 __global__ void __launch_bounds__(256)
     k_sparse_aa_even(T* __restrict__ f, int64_t n, int64_t start, int64_t count, double omega,
                      T F0, T F1, T F2) {
@@ -31,15 +50,15 @@ __global__ void __launch_bounds__(256)
   if (i >= start + count) return;
   T d[Q];
 #pragma unroll
-  for (int s = 0; s < Q; s++) d[s] = f[pidx<Q, LAYOUT>(n, Lattice<Q>::opp(s), i)];
+  for (int s = 0; s < Q; s++) d[s] = ldn<NT>(f + pidx<Q, LAYOUT>(n, Lattice<Q>::opp(s), i));
   const T F[3] = {F0, F1, F2};
   collide_ref<Q, T, false>(d, d, omega, F, 0.0);
 #pragma unroll
-  for (int s = 0; s < Q; s++) f[pidx<Q, LAYOUT>(n, s, i)] = d[s];
+  for (int s = 0; s < Q; s++) stn<NT>(f + pidx<Q, LAYOUT>(n, s, i), d[s]);
 }
 
 // AA odd: src/kernels/KernelAA_SOA_Base.cpp:257-342 (AOS: KernelAA_AOS_Base.cpp:258-342)
-template <int Q, typename T, int LAYOUT>
+template <int Q, typename T, int LAYOUT, bool NT>
 __global__ void __launch_bounds__(256)
     k_sparse_aa_odd(T* __restrict__ f, const int32_t* __restrict__ adj, int64_t n, int64_t start,
                     int64_t count, double omega, T F0, T F1, T F2) {
@@ -49,7 +68,7 @@ __global__ void __launch_bounds__(256)
   int32_t loc[Q];
 #pragma unroll
   for (int s = 0; s < Q; s++) {
-    loc[s] = adj[pidx<Q, LAYOUT>(n, s, i)];
+    loc[s] = ldn<NT>(adj + pidx<Q, LAYOUT>(n, s, i));
     d[s] = f[loc[s]];
   }
   const T F[3] = {F0, F1, F2};
@@ -59,7 +78,7 @@ __global__ void __launch_bounds__(256)
 }
 
 // AB push: src/kernels/KernelAB_SOA_Base.cpp:111-186 (AOS: KernelAB_AOS_Base.cpp:108-182)
-template <int Q, typename T, int LAYOUT>
+template <int Q, typename T, int LAYOUT, bool NT>
 __global__ void __launch_bounds__(256)
     k_sparse_ab_push(const T* __restrict__ src, T* __restrict__ tgt,
                      const int32_t* __restrict__ adj, int64_t n, int64_t start, int64_t count,
@@ -68,15 +87,15 @@ __global__ void __launch_bounds__(256)
   if (i >= start + count) return;
   T d[Q];
 #pragma unroll
-  for (int s = 0; s < Q; s++) d[s] = src[pidx<Q, LAYOUT>(n, s, i)];
+  for (int s = 0; s < Q; s++) d[s] = ldn<NT>(src + pidx<Q, LAYOUT>(n, s, i));
   const T F[3] = {F0, F1, F2};
   collide_ref<Q, T, false>(d, d, omega, F, 0.0);
 #pragma unroll
-  for (int s = 0; s < Q; s++) tgt[adj[pidx<Q, LAYOUT>(n, s, i)]] = d[s];
+  for (int s = 0; s < Q; s++) tgt[ldn<NT>(adj + pidx<Q, LAYOUT>(n, s, i))] = d[s];
 }
 
 // AB pull: src/kernels/KernelAB_SOA_Pull.cpp:113-194
-template <int Q, typename T, int LAYOUT>
+template <int Q, typename T, int LAYOUT, bool NT>
 __global__ void __launch_bounds__(256)
     k_sparse_ab_pull(const T* __restrict__ src, T* __restrict__ tgt,
                      const int32_t* __restrict__ adj, int64_t n, int64_t start, int64_t count,
@@ -85,15 +104,15 @@ __global__ void __launch_bounds__(256)
   if (i >= start + count) return;
   T d[Q];
 #pragma unroll
-  for (int s = 0; s < Q; s++) d[s] = src[adj[pidx<Q, LAYOUT>(n, s, i)]];
+  for (int s = 0; s < Q; s++) d[s] = src[ldn<NT>(adj + pidx<Q, LAYOUT>(n, s, i))];
   const T F[3] = {F0, F1, F2};
   collide_ref<Q, T, false>(d, d, omega, F, 0.0);
 #pragma unroll
-  for (int s = 0; s < Q; s++) tgt[pidx<Q, LAYOUT>(n, s, i)] = d[s];
+  for (int s = 0; s < Q; s++) stn<NT>(tgt + pidx<Q, LAYOUT>(n, s, i), d[s]);
 }
 
 // PropagationAA::initialize (slot opp(s) := w_s) and PropagationAB::initialize (slot s := w_s).
-template <int Q, typename T, int LAYOUT>
+template <int Q, typename T, int LAYOUT, bool NT>
 __global__ void __launch_bounds__(256) k_sparse_init_rest(T* __restrict__ f, int64_t n, int aa) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -115,20 +134,36 @@ struct lbm_sparse {
   double omega = 1.0;
   double force[3] = {1e-6, 0.0, 0.0};
   int32_t* adj = nullptr;  // device copy of the adjacency
+  // launch shapes: the AA even step (all node-aligned) and the other steps
+  unsigned even_bx = 256, step_bx = 256;
+  size_t even_lds = 0, step_lds = 0;
+  bool nt = true;  // non-temporal node-aligned accesses (arrays beyond the memory-side cache)
 };
 
 namespace {
 
 using namespace lbm;
 
-inline unsigned grid_for(int64_t count) { return (unsigned)((count + 255) / 256); }
+inline unsigned grid_for(int64_t count, unsigned bx = 256) {
+  return (unsigned)((count + bx - 1) / bx);
+}
+
+struct Shape {
+  unsigned grid, block;
+  size_t lds;
+};
 
 #define LBM_SPARSE_KERNEL_FN(name)                                                         \
   template <int Q, typename T, int L>                                                      \
   struct name##_fn {                                                                       \
     template <typename... A>                                                               \
-    static void launch(unsigned grid, A... a) {                                            \
-      hipLaunchKernelGGL((lbm::name<Q, T, L>), dim3(grid), dim3(256), 0, 0, a...);        \
+    static void launch(Shape sh, bool nt, A... a) {                                        \
+      if (nt)                                                                              \
+        hipLaunchKernelGGL((lbm::name<Q, T, L, true>), dim3(sh.grid), dim3(sh.block),      \
+                           sh.lds, 0, a...);                                               \
+      else                                                                                 \
+        hipLaunchKernelGGL((lbm::name<Q, T, L, false>), dim3(sh.grid), dim3(sh.block),     \
+                           sh.lds, 0, a...);                                               \
     }                                                                                      \
   };
 LBM_SPARSE_KERNEL_FN(k_sparse_aa_even)
@@ -139,14 +174,49 @@ LBM_SPARSE_KERNEL_FN(k_sparse_init_rest)
 #undef LBM_SPARSE_KERNEL_FN
 
 template <template <int, typename, int> class K, typename T, typename... A>
-void launch_qt(const lbm_sparse* h, unsigned grid, A... a) {
+void launch_qt(const lbm_sparse* h, Shape sh, A... a) {
   if (h->q == 19) {
-    if (h->layout == LBM_LAYOUT_SOA) K<19, T, LBM_LAYOUT_SOA>::launch(grid, a...);
-    else K<19, T, LBM_LAYOUT_AOS>::launch(grid, a...);
+    if (h->layout == LBM_LAYOUT_SOA) K<19, T, LBM_LAYOUT_SOA>::launch(sh, h->nt, a...);
+    else K<19, T, LBM_LAYOUT_AOS>::launch(sh, h->nt, a...);
   } else {
-    if (h->layout == LBM_LAYOUT_SOA) K<27, T, LBM_LAYOUT_SOA>::launch(grid, a...);
-    else K<27, T, LBM_LAYOUT_AOS>::launch(grid, a...);
+    if (h->layout == LBM_LAYOUT_SOA) K<27, T, LBM_LAYOUT_SOA>::launch(sh, h->nt, a...);
+    else K<27, T, LBM_LAYOUT_AOS>::launch(sh, h->nt, a...);
   }
+}
+
+Shape even_shape(const lbm_sparse* h, int64_t count) {
+  return {grid_for(count, h->even_bx), h->even_bx, h->even_lds};
+}
+Shape step_shape(const lbm_sparse* h, int64_t count) {
+  return {grid_for(count, h->step_bx), h->step_bx, h->step_lds};
+}
+
+// Launch shapes, fixed at create.  The AA even step is all node-aligned, like
+// the dense engine's even step, which measured fastest with small workgroups
+// at capped occupancy; the other steps keep 256-wide workgroups.  Environment
+// overrides (tuning): LBM_SPARSE_EVEN_BLOCK_X, LBM_SPARSE_EVEN_BLOCKS_PER_CU,
+// LBM_SPARSE_BLOCK_X, LBM_SPARSE_BLOCKS_PER_CU.
+void set_shapes(lbm_sparse* h) {
+  auto env_int = [](const char* name, int dflt) {
+    const char* e = std::getenv(name);
+    return e ? std::atoi(e) : dflt;
+  };
+  auto clamp_bx = [](int bx) { return (unsigned)(bx < 64 ? 64 : (bx > 256 ? 256 : bx / 64 * 64)); };
+  auto lds_for = [](int k) { return k > 0 ? (size_t)(160 * 1024 / k) & ~(size_t)1023 : (size_t)0; };
+  // footprint of one step: populations (two arrays for AB) and adjacency,
+  // against the 256 MiB memory-side cache of the MI355X (LBM_SPARSE_NT=0|1
+  // overrides)
+  const size_t esize = h->storage == LBM_STORE_F64 ? 8 : 4;
+  const size_t nq = (size_t)h->n * h->q;
+  const size_t bytes = nq * (esize * (h->adjacency_mode == 1 ? 1 : 2) + 4);
+  h->nt = env_int("LBM_SPARSE_NT", bytes > ((size_t)256 << 20) ? 1 : 0) != 0;
+  // the small capped even shape is for grids that stream from HBM; cached
+  // grids are small enough that the cap leaves the GPU partly idle
+  const bool big = h->nt && h->storage == LBM_STORE_F64;
+  h->even_bx = clamp_bx(env_int("LBM_SPARSE_EVEN_BLOCK_X", big ? 128 : 256));
+  h->even_lds = lds_for(env_int("LBM_SPARSE_EVEN_BLOCKS_PER_CU", big ? 4 : 0));
+  h->step_bx = clamp_bx(env_int("LBM_SPARSE_BLOCK_X", 256));
+  h->step_lds = lds_for(env_int("LBM_SPARSE_BLOCKS_PER_CU", 0));
 }
 
 lbm_status check_range(const lbm_sparse* h, int64_t start, int64_t count) {
@@ -185,6 +255,7 @@ lbm_status lbm_sparse_create(const lbm_sparse_config* cfg, lbm_sparse** out) {
   h->adjacency_mode = cfg->adjacency_mode;
   h->omega = cfg->omega;
   for (int d = 0; d < 3; d++) h->force[d] = cfg->force[d];
+  set_shapes(h);
   if (hipMalloc(&h->adj, sizeof(int32_t) * nq) != hipSuccess) {
     delete h;
     set_error("lbm_sparse_create: hipMalloc of the adjacency failed");
@@ -212,7 +283,7 @@ lbm_status lbm_sparse_aa_even(lbm_sparse* h, void* dstrb, int64_t start, int64_t
   LBM_REQUIRE(h && dstrb, LBM_ERR_ARG, "lbm_sparse_aa_even: null pointer");
   if (lbm_status st = check_range(h, start, count)) return st;
   if (count == 0) return LBM_OK;
-  const unsigned g = grid_for(count);
+  const Shape g = even_shape(h, count);
   if (h->storage == LBM_STORE_F64)
     launch_qt<k_sparse_aa_even_fn, double>(h, g, (double*)dstrb, h->n, start, count, h->omega,
                                           h->force[0], h->force[1], h->force[2]);
@@ -230,7 +301,7 @@ lbm_status lbm_sparse_aa_odd(lbm_sparse* h, void* dstrb, int64_t start, int64_t 
               "lbm_sparse_aa_odd: AA needs the x - c_s adjacency (adjacency_mode 1)");
   if (lbm_status st = check_range(h, start, count)) return st;
   if (count == 0) return LBM_OK;
-  const unsigned g = grid_for(count);
+  const Shape g = step_shape(h, count);
   if (h->storage == LBM_STORE_F64)
     launch_qt<k_sparse_aa_odd_fn, double>(h, g, (double*)dstrb, (const int32_t*)h->adj, h->n,
                                          start, count, h->omega, h->force[0], h->force[1],
@@ -253,7 +324,7 @@ lbm_status lbm_sparse_ab(lbm_sparse* h, const void* src, void* tgt, int64_t star
               "lbm_sparse_ab: the reference has no AOS pull kernel");
   if (lbm_status st = check_range(h, start, count)) return st;
   if (count == 0) return LBM_OK;
-  const unsigned g = grid_for(count);
+  const Shape g = step_shape(h, count);
   const int32_t* adj = h->adj;
   if (h->storage == LBM_STORE_F64) {
     if (pull)
@@ -279,7 +350,7 @@ lbm_status lbm_sparse_ab(lbm_sparse* h, const void* src, void* tgt, int64_t star
 
 lbm_status lbm_sparse_init_rest(lbm_sparse* h, void* dstrb, int32_t aa) {
   LBM_REQUIRE(h && dstrb, LBM_ERR_ARG, "lbm_sparse_init_rest: null pointer");
-  const unsigned g = grid_for(h->n);
+  const Shape g{grid_for(h->n), 256, 0};
   if (h->storage == LBM_STORE_F64)
     launch_qt<k_sparse_init_rest_fn, double>(h, g, (double*)dstrb, h->n, (int)aa);
   else

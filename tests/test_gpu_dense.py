@@ -59,8 +59,13 @@ CASES = [(q, pat, lay, masked) for q in (19, 27) for pat in ("aa", "push", "pull
          for lay in ("soa", "aos") for masked in (False, True)]
 
 
+@pytest.mark.parametrize("nt", ["0", "1"])
 @pytest.mark.parametrize("q,pattern,layout,masked", CASES)
-def test_dense_f64_bitwise_vs_oracle(q, pattern, layout, masked):
+def test_dense_f64_bitwise_vs_oracle(q, pattern, layout, masked, nt, monkeypatch):
+    """Every pattern, layout and lattice, with and without solids and walls,
+    with cached (nt 0, the default for grids that fit the memory-side cache)
+    and streaming (nt 1) kernels: FP64 bit for bit with the oracle."""
+    monkeypatch.setenv("LBM_DENSE_NT", nt)
     nx, ny, nz = 70, 12, 9  # nx > 64: rows span more than one wavefront
     solid = None
     periodic = (1, 1, 1)
@@ -133,9 +138,11 @@ def test_dense_masked_staged_gather(q, periodic, storage, monkeypatch):
     """Rows of 128 cells with 30% solid cells and walls (including a
     non-periodic x face): the pull step, and the push step run as a gather,
     take the LDS-staged kernel with bounce-back reads (staged mode 2; the
-    default runs masked gathers unstaged).  FP64 equals the oracle bit for bit
+    default runs masked gathers unstaged; push is forced through the gather,
+    which by default applies above 1 GiB).  FP64 equals the oracle bit for bit
     with the solid fill on and off; FP32 deviations within 1e-7."""
     monkeypatch.setenv("LBM_DENSE_STAGED", "2")
+    monkeypatch.setenv("LBM_DENSE_PUSH_GATHER", "1")
     nx, ny, nz = 128, 10, 7
     solid = (O.uniform01(41, np.arange(nx * ny * nz, dtype=np.uint64)) < 0.3).astype(np.uint8)
     geom = O.DenseGeometry(nx, ny, nz, solid, periodic=periodic)
@@ -211,16 +218,19 @@ def test_dense_pressure_jump_vs_oracle(q, pattern, case, monkeypatch):
     fc = seeded_fc(geom, q, seed=37)
     omega, force, jump = 1 / 0.7, (1e-6, 0.0, 0.0), 3e-3
     want = oracle_run_jump(geom, q, fc, pattern, 7, omega, force, jump)
-    print(f"[dense] jump q={q} {pattern} {case}", flush=True)
-    with DenseLattice(nx, ny, nz, q=q, pattern="aa" if pattern == "aa" else "ab",
-                      pull=pattern == "pull", storage="f64", periodic=periodic, omega=omega,
-                      force=force, solid=solid, density_jump_x=jump) as lat:
-        lat.set_state(dense_canonical(geom, q, fc))
-        lat.step(4)
-        lat.set_state(lat.get_state())
-        lat.step(3)
-        got = lat.get_state()[:, geom.cell_of_fluid]
-    assert np.array_equal(got, want)
+    # masked push: both the scatter kernel and the gather one step out of phase
+    for gather in (("0", "1") if pattern == "push" and solid is not None else ("0",)):
+        monkeypatch.setenv("LBM_DENSE_PUSH_GATHER", gather)
+        print(f"[dense] jump q={q} {pattern} {case} gather={gather}", flush=True)
+        with DenseLattice(nx, ny, nz, q=q, pattern="aa" if pattern == "aa" else "ab",
+                          pull=pattern == "pull", storage="f64", periodic=periodic, omega=omega,
+                          force=force, solid=solid, density_jump_x=jump) as lat:
+            lat.set_state(dense_canonical(geom, q, fc))
+            lat.step(4)
+            lat.set_state(lat.get_state())
+            lat.step(3)
+            got = lat.get_state()[:, geom.cell_of_fluid]
+        assert np.array_equal(got, want), gather
 
 
 def test_dense_pressure_jump_f32_and_mass():
@@ -267,12 +277,14 @@ STAGED = [(q, pat, st, force) for q in (19, 27) for pat in ("aa", "push", "pull"
 
 
 @pytest.mark.parametrize("q,pattern,storage,force", STAGED)
-def test_dense_staged_rows(q, pattern, storage, force, monkeypatch):
+@pytest.mark.parametrize("nt", ["0", "1"])
+def test_dense_staged_rows(q, pattern, storage, force, nt, monkeypatch):
     """Fully periodic SOA boxes whose rows are a multiple of 64 cells run the
     LDS-staged gather/scatter kernels (one workgroup per x-row; staged mode 2,
     so the FP16 variants, unstaged by default, are covered too).  FP64 equals
     the oracle bit for bit; FP32/FP16 deviation storage within its precision."""
     monkeypatch.setenv("LBM_DENSE_STAGED", "2")
+    monkeypatch.setenv("LBM_DENSE_NT", nt)
     nx, ny, nz = 128, 6, 5
     geom = O.DenseGeometry(nx, ny, nz)
     fc = seeded_fc(geom, q, seed=12)
