@@ -14,7 +14,7 @@ Each prints one JSON line per run.
   porous     D3Q19 AB push, FP32 deviation storage, 384^3 packed spheres of radius 6-16 added
              until 35% solid (seed 6), periodic, inlet/outlet pressure drop 1e-4 along x (the
              pressure-drop boundary: density jump 3e-4 across the periodic x boundary), 3000
-             steps; SOA and AOS.
+             steps; SOA and AOS, and an FP64 SOA run for the field-by-field error of FP32.
   precision  D3Q27 AA 512^3 Taylor-Green (u0 = 0.05, tau = 0.51), 2000 steps, FP64 vs FP32 / FP16
              deviation storage field by field (precision_study.py).
 
@@ -156,21 +156,31 @@ def porous(args):
     capi.add_spheres(solid, n, n, n, 6.0, 16.0, 0.35, 6, periodic=(1, 1, 1))
     steps = args.steps or 3000
     dp = 1e-4
-    for layout in ("soa", "aos"):
-        with capi.DenseLattice(n, n, n, q=19, pattern="ab", layout=layout, storage="f32s",
+    fluid = solid == 0
+    x = np.arange(n ** 3) % n
+    fields = {}
+    for layout, storage in (("soa", "f32s"), ("aos", "f32s"), ("soa", "f64")):
+        with capi.DenseLattice(n, n, n, q=19, pattern="ab", layout=layout, storage=storage,
                                periodic=(1, 1, 1), omega=1.0, solid=solid,
                                density_jump_x=3.0 * dp) as lat:
             lat.init_tgv(0.0, 0.0, 1)
             r = _time_dense(lat, steps, 10)
             rho, u = lat.moments()
-        fluid = solid == 0
-        x = np.arange(n ** 3) % n
-        r.update({"workload": f"D3Q19 AB push {layout.upper()} FP32-deviation 384^3 porous, 35% "
-                              f"spheres r 6-16 (seed 6), inlet/outlet pressure drop 1e-4 along x",
-                  "layout": layout, "solid_fraction": round(float((solid != 0).mean()), 4),
+        r.update({"workload": f"D3Q19 AB push {layout.upper()} "
+                              f"{'FP32-deviation' if storage == 'f32s' else 'FP64'} 384^3 porous, "
+                              f"35% spheres r 6-16 (seed 6), inlet/outlet pressure drop 1e-4 along x",
+                  "layout": layout, "storage": storage,
+                  "solid_fraction": round(float((solid != 0).mean()), 4),
                   "mean_ux": float(u[fluid, 0].mean()),
                   "rho_inlet_minus_outlet": float(rho[fluid & (x == 0)].mean()
                                                   - rho[fluid & (x == n - 1)].mean())})
+        if layout == "soa":
+            fields[storage] = (rho[fluid], u[fluid])
+        if storage == "f64":
+            (r32, u32), (r64, u64) = fields["f32s"], fields["f64"]
+            r["f32s_vs_f64"] = {"rel_l2_u": float(np.linalg.norm(u32 - u64) / np.linalg.norm(u64)),
+                                "rel_l2_rho_dev": float(np.linalg.norm(r32 - r64)
+                                                        / np.linalg.norm(r64 - 1.0))}
         _emit(r)
 
 
