@@ -175,6 +175,27 @@ lbm_status lbm_dense_moments(lbm_dense* h, double* rho, double* u);
  * LBM_ERR_NUMERIC if the sum is not finite. */
 lbm_status lbm_dense_mass(lbm_dense* h, double* mass);
 lbm_status lbm_dense_synchronize(lbm_dense* h);
+/* Streams and graphs.  No counterpart in the reference (whose propagation runs
+ * on the host); for callers that order the engine with their own device work
+ * or replay its time loop as a HIP graph.
+ *
+ * lbm_dense_set_stream: enqueue every later call on `stream` (a hipStream_t;
+ * NULL returns to the handle's own stream).  Waits for the work already
+ * enqueued on the previous stream.  The stream must outlive its use by the
+ * handle.
+ *
+ * lbm_dense_graph_create: capture `nsteps` time steps (even, > 0) as a HIP graph
+ * of kernel launches only (no memory-set or copy nodes).  Nothing runs and the
+ * handle's state is unchanged.  lbm_dense_graph_launch replays it `nlaunch`
+ * times on the handle's current stream and advances time_step by
+ * nlaunch * nsteps; it returns LBM_ERR_ARG when the handle is an odd number of
+ * steps away from where it was at capture.  A graph belongs to its handle:
+ * destroy the graph first. */
+typedef struct lbm_dense_graph lbm_dense_graph;
+lbm_status lbm_dense_set_stream(lbm_dense* h, void* stream);
+lbm_status lbm_dense_graph_create(lbm_dense* h, int32_t nsteps, lbm_dense_graph** out);
+lbm_status lbm_dense_graph_launch(lbm_dense_graph* g, int32_t nlaunch);
+lbm_status lbm_dense_graph_destroy(lbm_dense_graph* g);
 
 /* ---------------------------------------------------------------- discrete adjoint
  * No counterpart in the reference.  Gradients of an objective of the final

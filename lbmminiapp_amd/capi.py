@@ -31,6 +31,8 @@ EXPORTED = [
     "lbm_dense_create", "lbm_dense_destroy", "lbm_dense_get_info", "lbm_dense_set_state",
     "lbm_dense_get_state", "lbm_dense_init_equilibrium", "lbm_dense_init_tgv", "lbm_dense_step",
     "lbm_dense_run_timed", "lbm_dense_moments", "lbm_dense_mass", "lbm_dense_synchronize",
+    "lbm_dense_set_stream", "lbm_dense_graph_create", "lbm_dense_graph_launch",
+    "lbm_dense_graph_destroy",
     "lbm_geometry_cylinder", "lbm_geometry_channel", "lbm_geometry_add_spheres",
     "lbm_field_smooth_random", "lbm_copy_bandwidth",
     "lbm_adjoint_create", "lbm_adjoint_destroy", "lbm_adjoint_set_initial_state",
@@ -107,6 +109,8 @@ def load(path: str = LIB_PATH):
         "lbm_dense_init_equilibrium": [P, P, P], "lbm_dense_init_tgv": [P, D, D, ctypes.c_uint64],
         "lbm_dense_step": [P, I32], "lbm_dense_run_timed": [P, I32, P],
         "lbm_dense_moments": [P, P, P], "lbm_dense_mass": [P, P], "lbm_dense_synchronize": [P],
+        "lbm_dense_set_stream": [P, P], "lbm_dense_graph_create": [P, I32, P],
+        "lbm_dense_graph_launch": [P, I32], "lbm_dense_graph_destroy": [P],
         "lbm_copy_bandwidth": [ctypes.c_size_t, I32, P],
         "lbm_geometry_cylinder": [D, P, P], "lbm_geometry_channel": [I32, I32, I32, P],
         "lbm_geometry_add_spheres": [I32, I32, I32, P, D, D, D, ctypes.c_uint64, P, P],
@@ -338,10 +342,51 @@ class DenseLattice:
     def synchronize(self):
         check(load().lbm_dense_synchronize(self.h))
 
+    def set_stream(self, stream=None):
+        """Enqueue later calls on `stream` (a hipStream_t as an int, e.g.
+        torch.cuda.Stream().cuda_stream; None: the handle's own stream)."""
+        check(load().lbm_dense_set_stream(self.h, ctypes.c_void_p(stream or None)))
+
+    def graph(self, nsteps) -> "DenseGraph":
+        """Capture `nsteps` (even) time steps as a HIP graph (nothing runs)."""
+        return DenseGraph(self, nsteps)
+
     def close(self):
         if self.h:
             check(load().lbm_dense_destroy(self.h))
             self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DenseGraph:
+    """An even number of time steps captured as a HIP graph (lbm_dense_graph_*).
+    Keeps its lattice alive; close it before the lattice."""
+
+    def __init__(self, lattice, nsteps):
+        self.lattice = lattice
+        self.nsteps = int(nsteps)
+        self.g = ctypes.c_void_p()
+        check(load().lbm_dense_graph_create(lattice.h, self.nsteps, ctypes.byref(self.g)))
+
+    def launch(self, n=1):
+        """Replay n times on the lattice's current stream (asynchronous)."""
+        check(load().lbm_dense_graph_launch(self.g, int(n)))
+
+    def close(self):
+        if self.g:
+            check(load().lbm_dense_graph_destroy(self.g))
+            self.g = ctypes.c_void_p()
 
     def __enter__(self):
         return self

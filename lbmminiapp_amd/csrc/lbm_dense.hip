@@ -446,7 +446,7 @@ void free_all(lbm_dense* h) {
   if (h->partial) (void)hipFree(h->partial);
   if (h->e0) (void)hipEventDestroy(h->e0);
   if (h->e1) (void)hipEventDestroy(h->e1);
-  if (h->stream) (void)hipStreamDestroy(h->stream);
+  if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
   delete h;
 }
 
@@ -538,11 +538,12 @@ lbm_status dense_create(const lbm_dense_config* cfg, lbm_dense** out, bool push_
     free_all(h);
     return s;
   };
-  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+  if (hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&h->e0) != hipSuccess || hipEventCreate(&h->e1) != hipSuccess) {
     set_error("lbm_dense_create: stream/event creation failed");
     return fail(LBM_ERR_HIP);
   }
+  h->stream = h->own_stream;
   const size_t fbytes = (size_t)h->ncell * h->q * h->esize;
   const int narrays = h->pattern == LBM_PATTERN_AA ? 1 : 2;
   for (int i = 0; i < narrays; i++)
@@ -759,6 +760,99 @@ lbm_status lbm_dense_run_timed(lbm_dense* h, int32_t nsteps, double* elapsed_ms)
   float ms = 0.f;
   LBM_HIP_CHECK(hipEventElapsedTime(&ms, h->e0, h->e1));
   *elapsed_ms = ms;
+  return LBM_OK;
+}
+
+// Every call on the handle enqueues on h->stream.  Switching waits for the
+// work already on the old stream, so the order of calls is kept.
+lbm_status lbm_dense_set_stream(lbm_dense* h, void* stream) {
+  LBM_REQUIRE(h, LBM_ERR_ARG, "lbm_dense_set_stream: null handle");
+  LBM_HIP_CHECK(hipStreamSynchronize(h->stream));
+  h->stream = stream ? (hipStream_t)stream : h->own_stream;
+  return LBM_OK;
+}
+
+// A captured run of an even number of steps.  After an even number of steps
+// the AA array is back in the convention it started in and the AB pair has
+// swapped twice, so one replay continues the run from any state whose phase
+// (t mod 2 for AA, the current array for AB) equals the phase at capture.
+// The kernels take the handle's device pointers, which stay fixed for the
+// handle's lifetime; capture records kernel launches only.
+struct lbm_dense_graph {
+  lbm_dense* h = nullptr;
+  int32_t nsteps = 0;
+  int phase = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+static int dense_phase(const lbm_dense* h) {
+  return h->pattern == LBM_PATTERN_AA ? (int)(h->t % 2) : h->cur;
+}
+
+lbm_status lbm_dense_graph_destroy(lbm_dense_graph* g) {
+  if (!g) return LBM_OK;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
+  return LBM_OK;
+}
+
+lbm_status lbm_dense_graph_create(lbm_dense* h, int32_t nsteps, lbm_dense_graph** out) {
+  LBM_REQUIRE(h && out, LBM_ERR_ARG, "lbm_dense_graph_create: null pointer");
+  LBM_REQUIRE(nsteps > 0 && nsteps % 2 == 0, LBM_ERR_ARG,
+              "lbm_dense_graph_create: nsteps must be even and positive (got %d)", nsteps);
+  *out = nullptr;
+  auto* g = new lbm_dense_graph;
+  g->h = h;
+  g->nsteps = nsteps;
+  g->phase = dense_phase(h);
+  // capture on a stream of its own: the caller's stream may be the null
+  // stream, which cannot be captured, and nothing captured runs now
+  hipStream_t cap = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete g;
+    set_error("lbm_dense_graph_create: %s", hipGetErrorString(e));
+    return LBM_ERR_HIP;
+  }
+  const hipStream_t user = h->stream;
+  const int64_t t0 = h->t;
+  const int cur0 = h->cur;
+  h->stream = cap;
+  lbm_status st = LBM_OK;
+  e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    st = enqueue_steps(h, nsteps);
+    hipGraph_t graph = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(cap, &graph);
+    g->graph = graph;
+    if (st == LBM_OK && e2 != hipSuccess) e = e2;
+  }
+  h->stream = user;
+  h->t = t0;
+  h->cur = cur0;
+  (void)hipStreamDestroy(cap);
+  if (st == LBM_OK && e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  if (st != LBM_OK || e != hipSuccess) {
+    if (st == LBM_OK) set_error("lbm_dense_graph_create: %s", hipGetErrorString(e));
+    lbm_dense_graph_destroy(g);
+    return st != LBM_OK ? st : LBM_ERR_HIP;
+  }
+  *out = g;
+  return LBM_OK;
+}
+
+lbm_status lbm_dense_graph_launch(lbm_dense_graph* g, int32_t nlaunch) {
+  LBM_REQUIRE(g && nlaunch >= 0, LBM_ERR_ARG, "lbm_dense_graph_launch: bad arguments");
+  lbm_dense* h = g->h;
+  LBM_REQUIRE(dense_phase(h) == g->phase, LBM_ERR_ARG,
+              "lbm_dense_graph_launch: the handle is an odd number of steps away from the "
+              "state the graph was captured in");
+  for (int32_t i = 0; i < nlaunch; i++) {
+    LBM_HIP_CHECK(hipGraphLaunch(g->exec, h->stream));
+    h->t += g->nsteps;
+  }
   return LBM_OK;
 }
 

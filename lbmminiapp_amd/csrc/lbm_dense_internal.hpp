@@ -23,7 +23,8 @@ struct lbm_dense {
   double* gamma_field = nullptr;
   double* partial = nullptr;  // mass partial sums + result
   int npartial = 1024;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;      // the stream every call enqueues on
+  hipStream_t own_stream = nullptr;  // the handle's own stream (stream unless the caller set one)
   hipEvent_t e0 = nullptr, e1 = nullptr;
   lbm::DenseArgs args{};
   size_t device_bytes = 0;
