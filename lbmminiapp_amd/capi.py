@@ -343,8 +343,8 @@ class DenseLattice:
         check(load().lbm_dense_synchronize(self.h))
 
     def set_stream(self, stream=None):
-        """Enqueue later calls on `stream` (a hipStream_t as an int, e.g.
-        torch.cuda.Stream().cuda_stream; None: the handle's own stream)."""
+        """Enqueue later calls on `stream` (a hipStream_t as an int, created by
+        the HIP runtime this library is bound to; None: the handle's own stream)."""
         check(load().lbm_dense_set_stream(self.h, ctypes.c_void_p(stream or None)))
 
     def graph(self, nsteps) -> "DenseGraph":

@@ -379,9 +379,14 @@ int canon_mode(const lbm_dense* h) {
   return h->pull ? C_GATHER : C_LOCAL;
 }
 
+// kernels with the Guo forcing terms: a nonzero force, or the policy set at create
+bool use_force_terms(const lbm_dense* h) {
+  return h->force_terms || h->args.F0 != 0.0 || h->args.F1 != 0.0 || h->args.F2 != 0.0;
+}
+
 DenseLaunchFn pick_step(const lbm_dense* h, int kind) {
   const bool mask = h->links != nullptr;
-  const bool force = h->args.F0 != 0.0 || h->args.F1 != 0.0 || h->args.F2 != 0.0;
+  const bool force = use_force_terms(h);
   return h->q == 19 ? pick_dense19(h->storage, h->layout, kind, mask, force, h->nt)
                     : pick_dense27(h->storage, h->layout, kind, mask, force, h->nt);
 }
@@ -401,7 +406,7 @@ DenseLaunchFn pick_staged(const lbm_dense* h, int kind) {
   if (mode == 0 || h->layout != LBM_LAYOUT_SOA || h->nx % 64 != 0 ||
       h->nx > kDenseMaxBlock || (mode < 2 && (h->storage == LBM_STORE_F16S || mask)))
     return nullptr;
-  const bool force = h->args.F0 != 0.0 || h->args.F1 != 0.0 || h->args.F2 != 0.0;
+  const bool force = use_force_terms(h);
   return h->q == 19 ? pick_staged19(h->storage, kind, force, h->nx, mask, h->lds_max, h->nt)
                     : pick_staged27(h->storage, kind, force, h->nx, mask, h->lds_max, h->nt);
 }
@@ -632,6 +637,18 @@ lbm_status dense_create(const lbm_dense_config* cfg, lbm_dense** out, bool push_
   a.scale = (float)scale;
   a.inv_scale = (float)(1.0 / scale);
   a.djump = cfg->density_jump_x;
+  // With F = 0 the Guo terms are exact zeros, so the kernels that evaluate
+  // them give the same bits as the specialised ones.  For D3Q19 FP64 AA on
+  // SOA boxes without solids they also measured faster (512^3, alternating
+  // launches on one array: odd step 7.46 vs 7.98 ms, even 6.88 vs 6.94 ms);
+  // in every other case measured (FP32/FP16 deviations, D3Q27, AB) they were
+  // equal or slower.  LBM_DENSE_FORCE_TERMS=0|1 overrides.
+  {
+    const bool dflt = h->q == 19 && h->storage == LBM_STORE_F64 && h->pattern == LBM_PATTERN_AA &&
+                      h->layout == LBM_LAYOUT_SOA && h->links == nullptr;
+    const char* e = std::getenv("LBM_DENSE_FORCE_TERMS");
+    h->force_terms = e ? e[0] == '1' : dflt;
+  }
   // rest state, as PropagationAA/AB::initialize (src/propagation/PropagationAA.cpp:60-123)
   if ((st = init_from(h, 0, nullptr, nullptr, 0.0, 0.0, 0))) return fail(st);
   if (h->pattern == LBM_PATTERN_AB &&

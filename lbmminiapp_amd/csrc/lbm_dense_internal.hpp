@@ -14,6 +14,7 @@ struct lbm_dense {
   size_t step_lds = 0, even_lds = 0;
   size_t lds_max = 65536;  // the device's LDS limit per workgroup
   bool nt = true;          // streaming (non-temporal) accesses in the time-step kernels
+  bool force_terms = false;  // time-step kernels with the Guo terms also at F = 0
   int64_t ncell = 0, nfluid = 0, t = 0;
   size_t esize = 8;
   void* f[2] = {nullptr, nullptr};

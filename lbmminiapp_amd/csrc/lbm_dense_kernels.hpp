@@ -301,8 +301,25 @@ __host__ __device__ constexpr int n_xshift() {
 // whose neighbour x - c_s is solid or outside then takes slot opp(s) at x
 // instead, solid nodes skip the collision and store zeros (a.fill_solid), and
 // all threads stay for the barrier.
-template <int Q, int STORE, int KIND, bool FORCE, bool MASK = false, bool NT = true>
-__global__ void __launch_bounds__(kDenseMaxBlock)
+// Waves per SIMD the staged kernel is compiled for.  A whole-row workgroup of
+// 512 threads is 8 wavefronts, 2 per SIMD, so a CU holds two of them only at
+// 4 waves per SIMD, i.e. at most 128 VGPRs.  D3Q19 FP64 needs 130 (144 with
+// the Guo terms) unbounded; bounded to 128 (7 spilled with the Guo terms), two
+// rows per CU overlap each other's barriers.  512^3 AA odd step, alternating
+// launches on one array: with the Guo terms 7.47-7.51 ms bounded vs 8.17-8.24
+// unbounded, without 7.83-7.95 vs 7.99-8.02.  Bounds of 5-8 waves spill more
+// and ran slower (8.8-18 ms).  Q27 spills heavily at 128 and ran 10-45%
+// slower, FP32/FP16 deviations gained nothing, so they stay unbounded
+// (1 = no constraint).
+template <int Q, int STORE>
+constexpr int staged_min_waves() {
+  return (Q == 19 && STORE == LBM_STORE_F64) ? 4 : 1;
+}
+
+template <int Q, int STORE, int KIND, bool FORCE, bool MASK = false, bool NT = true,
+          int WAVES = staged_min_waves<Q, STORE>()>
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, kDenseMaxBlock),
+                               amdgpu_waves_per_eu(WAVES)))
     k_dense_staged(const typename Storage<STORE>::T* f0, typename Storage<STORE>::T* f1,
                    DenseArgs a) {
   static_assert(!MASK || KIND == K_AB_PULL, "masked staging covers the pull gather");

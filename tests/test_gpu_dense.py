@@ -253,9 +253,13 @@ def test_dense_pressure_jump_f32_and_mass():
 @pytest.mark.parametrize("q", [19, 27])
 @pytest.mark.parametrize("pattern", ["aa", "push", "pull"])
 @pytest.mark.parametrize("storage", ["f64", "f32s"])
-def test_dense_zero_force_kernels(q, pattern, storage):
-    """F = 0 selects kernels without the Guo terms; for FP64 they equal the
-    oracle (which evaluates every Guo term) bit for bit."""
+@pytest.mark.parametrize("terms", ["0", "1"])
+def test_dense_zero_force_kernels(q, pattern, storage, terms, monkeypatch):
+    """F = 0 runs kernels without the Guo terms (LBM_DENSE_FORCE_TERMS=0) or
+    with them evaluated as exact zeros (=1, the default for D3Q19 FP64 AA on
+    open SOA boxes); for FP64 both equal the oracle (which evaluates every Guo
+    term) bit for bit."""
+    monkeypatch.setenv("LBM_DENSE_FORCE_TERMS", terms)
     nx, ny, nz = 66, 10, 8
     geom = O.DenseGeometry(nx, ny, nz)
     fc = seeded_fc(geom, q, seed=8)
@@ -270,6 +274,27 @@ def test_dense_zero_force_kernels(q, pattern, storage):
         assert np.array_equal(got, want)
     else:
         assert np.abs(got - want).max() < 1e-7
+
+
+@pytest.mark.parametrize("terms", ["0", "1", None])
+def test_dense_staged_aa_f64_force_terms_policy(terms, monkeypatch):
+    """D3Q19 FP64 AA on a periodic SOA box with rows of 128 (the staged odd
+    step, the default-policy case): at F = 0 the kernels with and without the
+    Guo terms, and the default choice, all equal the oracle bit for bit."""
+    if terms is not None:
+        monkeypatch.setenv("LBM_DENSE_FORCE_TERMS", terms)
+    else:
+        monkeypatch.delenv("LBM_DENSE_FORCE_TERMS", raising=False)
+    nx, ny, nz = 128, 6, 5
+    geom = O.DenseGeometry(nx, ny, nz)
+    fc = seeded_fc(geom, 19, seed=19)
+    omega, steps = 1 / 0.6, 6
+    with DenseLattice(nx, ny, nz, q=19, pattern="aa", storage="f64", omega=omega) as lat:
+        lat.set_state(fc)
+        lat.step(steps)
+        got = lat.get_state()
+    want = oracle_run(geom, 19, fc, "aa", steps, omega, (0.0, 0.0, 0.0))
+    assert np.array_equal(got, want)
 
 
 STAGED = [(q, pat, st, force) for q in (19, 27) for pat in ("aa", "push", "pull")

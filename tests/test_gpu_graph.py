@@ -2,6 +2,8 @@
 HIP graph and replayed on a caller's stream equal the same steps enqueued one
 by one, bit for bit, for every replay, with other device work, eager steps and
 synchronisations between the replays."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -36,56 +38,79 @@ def _lattice(case):
     return lat
 
 
+class _Stream:
+    """A non-blocking stream of the HIP runtime the library is bound to (found by
+    its soname once the library is loaded, so that the handle is valid there)."""
+
+    def __init__(self):
+        capi.load()
+        self.hip = ctypes.CDLL("libamdhip64.so.7")
+        self.hip.hipStreamCreateWithFlags.argtypes = [ctypes.c_void_p, ctypes.c_uint]
+        self.hip.hipStreamDestroy.argtypes = [ctypes.c_void_p]
+        self.hip.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
+        self.s = ctypes.c_void_p()
+        assert self.hip.hipStreamCreateWithFlags(ctypes.byref(self.s), 1) == 0  # hipStreamNonBlocking
+
+    def synchronize(self):
+        assert self.hip.hipStreamSynchronize(self.s) == 0
+
+    def close(self):
+        if self.s:
+            assert self.hip.hipStreamDestroy(self.s) == 0
+            self.s = ctypes.c_void_p()
+
+
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_graph_replays_equal_eager_steps(name, monkeypatch):
-    import torch
-
     case = CASES[name]
     if "env" in case:
         monkeypatch.setenv("LBM_DENSE_PUSH_GATHER", case["env"])
     print(f"[graph] {name} {NX}x{NY}x{NZ}", flush=True)
-    stream = torch.cuda.Stream(device="cuda:0")
-    with _lattice(case) as a, _lattice(case) as b:
-        b.set_stream(stream.cuda_stream)
-        with b.graph(4) as g:
-            # capture runs nothing and leaves the handle where it was
-            assert b.info()["time_step"] == 0
-            assert np.array_equal(b.get_state(), a.get_state())
-            work = torch.zeros(1 << 20, device="cuda:0")
-            for r in range(3):
+    stream = _Stream()
+    try:
+        with _lattice(case) as a, _lattice(case) as b:
+            # both handles on the caller's stream: a's eager steps are other
+            # device work ordered between b's replays
+            a.set_stream(stream.s.value)
+            b.set_stream(stream.s.value)
+            with b.graph(4) as g:
+                # capture runs nothing and leaves the handle where it was
+                assert b.info()["time_step"] == 0
+                assert np.array_equal(b.get_state(), a.get_state())
+                for r in range(3):
+                    g.launch(1)
+                    a.step(4)
+                    assert np.array_equal(b.get_state(), a.get_state()), f"replay {r}"
+                    b.step(2)
+                    a.step(2)
+                    stream.synchronize()
+                g.launch(2)
+                a.step(8)
+                assert np.array_equal(b.get_state(), a.get_state())
+                assert b.info()["time_step"] == a.info()["time_step"] == 26
+                # an odd number of steps away from the capture: refused, nothing enqueued
+                b.step(1)
+                a.step(1)
+                with pytest.raises(capi.LbmError):
+                    g.launch(1)
+                # a graph captured in the other phase
+                with b.graph(2) as g2:
+                    g2.launch(3)
+                    a.step(6)
+                    assert np.array_equal(b.get_state(), a.get_state())
+                b.step(1)
+                a.step(1)
                 g.launch(1)
                 a.step(4)
-                with torch.cuda.stream(stream):
-                    work.add_(1.0)
-                assert np.array_equal(b.get_state(), a.get_state()), f"replay {r}"
-                b.step(2)
-                a.step(2)
-                torch.cuda.synchronize()
-            g.launch(2)
-            a.step(8)
-            assert np.array_equal(b.get_state(), a.get_state())
-            assert b.info()["time_step"] == a.info()["time_step"] == 26
-            # an odd number of steps away from the capture: refused, nothing enqueued
-            b.step(1)
-            a.step(1)
-            with pytest.raises(capi.LbmError):
-                g.launch(1)
-            # a graph captured in the other phase
-            with b.graph(2) as g2:
-                g2.launch(3)
-                a.step(6)
                 assert np.array_equal(b.get_state(), a.get_state())
-            b.step(1)
-            a.step(1)
-            g.launch(1)
-            a.step(4)
+                assert abs(b.mass() - a.mass()) == 0.0
+            a.set_stream(None)
+            b.set_stream(None)
+            b.step(2)
+            a.step(2)
             assert np.array_equal(b.get_state(), a.get_state())
-            assert abs(b.mass() - a.mass()) == 0.0
-            assert float(work[0]) == 3.0
-        b.set_stream(None)
-        b.step(2)
-        a.step(2)
-        assert np.array_equal(b.get_state(), a.get_state())
+    finally:
+        stream.close()
 
 
 def test_graph_rejects_odd_step_counts():
