@@ -169,6 +169,44 @@ __device__ __forceinline__ bool leaves_box(const DenseArgs& a, int x, int y, int
          (!(p & 4u) && ((cz < 0 && z == 0) || (cz > 0 && z == a.nz - 1)));
 }
 
+// One node's collision, with the Brinkman term when a gamma field is given.
+// ONE: a single collision body with gamma = 0 where there is no field (the
+// factor 1 - 0 = 1 leaves every velocity bit unchanged, so the results are the
+// same); otherwise separate bodies with and without the Brinkman term.  The
+// single body needs fewer VGPRs but schedules differently; which is faster
+// depends on the kernel (one_collision_body).
+template <int Q, int STORE, bool FORCE, bool ONE>
+__device__ __forceinline__ void collide_cell(typename Storage<STORE>::C* d, double om,
+                                             const double* F, const double* gamma_field,
+                                             int64_t cell) {
+  if constexpr (ONE) {
+    const double gm = gamma_field ? gamma_field[cell] : 0.0;
+    if constexpr (STORE == LBM_STORE_F64) collide_ref<Q, double, true, FORCE>(d, d, om, F, gm);
+    else collide_shifted<Q, true, FORCE>(d, om, F, gm);
+  } else if constexpr (STORE == LBM_STORE_F64) {
+    if (gamma_field) collide_ref<Q, double, true, FORCE>(d, d, om, F, gamma_field[cell]);
+    else collide_ref<Q, double, false, FORCE>(d, d, om, F, 0.0);
+  } else {
+    if (gamma_field) collide_shifted<Q, true, FORCE>(d, om, F, gamma_field[cell]);
+    else collide_shifted<Q, false, FORCE>(d, om, F, 0.0);
+  }
+}
+
+// Single or separate collision bodies, from 512^3 timings of every kind with
+// alternating launches on one array (ratio one / two bodies):
+//   FP16 deviations: one everywhere (0.77-0.98; even steps 0.97-1.01)
+//   FP32 deviations: one for AA odd steps with forcing (0.81 D3Q19, 0.83-0.93
+//     D3Q27) and D3Q27 AA odd steps without (0.90); two elsewhere (D3Q19 odd
+//     without forcing 1.05, D3Q19 push 1.04-1.05, even and pull 1.00-1.03)
+//   FP64: two everywhere (D3Q19 odd with forcing 1.07, D3Q27 push 1.05,
+//     the rest 0.98-1.02)
+template <int Q, int STORE, int KIND, bool FORCE>
+constexpr bool one_collision_body() {
+  if (STORE == LBM_STORE_F16S) return true;
+  if (STORE == LBM_STORE_F32S) return KIND == K_AA_ODD && (FORCE || Q == 27);
+  return false;
+}
+
 // Blocks are one x-row segment of up to kDenseMaxBlock cells (a full row when
 // nx <= kDenseMaxBlock, so that every cache line shared by x-shifted accesses is
 // used by one workgroup).
@@ -185,7 +223,8 @@ constexpr int kDenseMaxBlock = 512;
 //                      the slot at T that streaming would have written
 //     T outside        fluid x bounces back; solid x stores a zero into that
 //                      bounce-back slot
-template <int Q, int STORE, int LAYOUT, int KIND, bool MASK, bool FORCE, bool NT = true>
+template <int Q, int STORE, int LAYOUT, int KIND, bool MASK, bool FORCE, bool NT = true,
+          bool ONE = one_collision_body<Q, STORE, KIND, FORCE>()>
 __global__ void __launch_bounds__(kDenseMaxBlock)
     k_dense(const typename Storage<STORE>::T* f0, typename Storage<STORE>::T* f1, DenseArgs a) {
   using C = typename Storage<STORE>::C;
@@ -239,13 +278,7 @@ __global__ void __launch_bounds__(kDenseMaxBlock)
 
     const double om = a.omega_field ? a.omega_field[cell] : a.omega;
     const double F[3] = {a.F0, a.F1, a.F2};
-    if constexpr (STORE == LBM_STORE_F64) {
-      if (a.gamma_field) collide_ref<Q, double, true, FORCE>(d, d, om, F, a.gamma_field[cell]);
-      else collide_ref<Q, double, false, FORCE>(d, d, om, F, 0.0);
-    } else {
-      if (a.gamma_field) collide_shifted<Q, true, FORCE>(d, om, F, a.gamma_field[cell]);
-      else collide_shifted<Q, false, FORCE>(d, om, F, 0.0);
-    }
+    collide_cell<Q, STORE, FORCE, ONE>(d, om, F, a.gamma_field, cell);
   }
 
   auto put = [&](int64_t idx, C v, bool aligned) {
@@ -317,7 +350,8 @@ constexpr int staged_min_waves() {
 }
 
 template <int Q, int STORE, int KIND, bool FORCE, bool MASK = false, bool NT = true,
-          int WAVES = staged_min_waves<Q, STORE>()>
+          int WAVES = staged_min_waves<Q, STORE>(),
+          bool ONE = one_collision_body<Q, STORE, KIND, FORCE>()>
 __global__ void __attribute__((amdgpu_flat_work_group_size(1, kDenseMaxBlock),
                                amdgpu_waves_per_eu(WAVES)))
     k_dense_staged(const typename Storage<STORE>::T* f0, typename Storage<STORE>::T* f1,
@@ -382,13 +416,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, kDenseMaxBlock),
   if (!solid) {
     const double om = a.omega_field ? a.omega_field[cell] : a.omega;
     const double F[3] = {a.F0, a.F1, a.F2};
-    if constexpr (STORE == LBM_STORE_F64) {
-      if (a.gamma_field) collide_ref<Q, double, true, FORCE>(d, d, om, F, a.gamma_field[cell]);
-      else collide_ref<Q, double, false, FORCE>(d, d, om, F, 0.0);
-    } else {
-      if (a.gamma_field) collide_shifted<Q, true, FORCE>(d, om, F, a.gamma_field[cell]);
-      else collide_shifted<Q, false, FORCE>(d, om, F, 0.0);
-    }
+    collide_cell<Q, STORE, FORCE, ONE>(d, om, F, a.gamma_field, cell);
   }
 
   if constexpr (SCATTER) {
