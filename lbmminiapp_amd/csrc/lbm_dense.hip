@@ -394,18 +394,22 @@ DenseLaunchFn pick_step(const lbm_dense* h, int kind) {
 // The LDS-staged kernels cover SOA grids whose rows fit one workgroup (nx a
 // multiple of 64, nx <= kDenseMaxBlock): every kind on fully periodic boxes,
 // and the pull gather also with solid cells and walls.  By default (staged
-// mode 1, set at create from LBM_DENSE_STAGED) they run on periodic boxes in
-// FP64 and FP32 deviation storage only, where they measured faster; the
-// unstaged kernels measured faster for FP16 rows (2.1 vs 2.2 ms per 512^3
-// D3Q19 odd step) and for masked gathers (35% solid porous pull at 384^3:
-// 20.4k vs 18.7k MLUPS; a 5% solid channel: equal).  Mode 0 never stages,
-// mode 2 stages every case covered (tuning and cross-checks).
+// mode 1, set at create from LBM_DENSE_STAGED) they run where they measured
+// faster (512^3 steps, two handles alternating in one process): periodic
+// boxes in FP64 and FP32 deviations, and FP16 deviation rows for the AB steps
+// (D3Q19 pull 1.90 vs 2.05 ms, push 1.86 vs 2.05; D3Q27 push 2.81 vs 3.27)
+// except the D3Q27 pull (3.12 vs 3.02 for the kernel alone).  The FP16 AA
+// step stays unstaged (D3Q19 1.94 vs 1.91 ms).  Masked gathers run unstaged
+// (35% solid porous pull at 384^3: 20.4k vs 18.7k MLUPS staged; a 5% solid
+// channel: equal).  Mode 0 never stages, mode 2 stages every case covered
+// (tuning and cross-checks).
 DenseLaunchFn pick_staged(const lbm_dense* h, int kind) {
   const int mode = h->staged_mode;
   const bool mask = h->links != nullptr;
-  if (mode == 0 || h->layout != LBM_LAYOUT_SOA || h->nx % 64 != 0 ||
-      h->nx > kDenseMaxBlock || (mode < 2 && (h->storage == LBM_STORE_F16S || mask)))
+  if (mode == 0 || h->layout != LBM_LAYOUT_SOA || h->nx % 64 != 0 || h->nx > kDenseMaxBlock)
     return nullptr;
+  const bool f16_staged = kind == K_AB_PUSH || (kind == K_AB_PULL && h->q == 19);
+  if (mode < 2 && (mask || (h->storage == LBM_STORE_F16S && !f16_staged))) return nullptr;
   const bool force = use_force_terms(h);
   return h->q == 19 ? pick_staged19(h->storage, kind, force, h->nx, mask, h->lds_max, h->nt)
                     : pick_staged27(h->storage, kind, force, h->nx, mask, h->lds_max, h->nt);
