@@ -45,8 +45,8 @@ __device__ __forceinline__ double full_value(typename Storage<STORE>::C v, int s
   else return Lattice<Q>::w(s) + (double)v;
 }
 
-template <int Q, int STORE, bool MASK>
-__global__ void __launch_bounds__(256)
+template <int Q, int STORE, bool MASK, int WAVES>
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, 256), amdgpu_waves_per_eu(WAVES)))
     k_adj_step(const void* fv, const void* lin, void* lout, double* __restrict__ grad_gamma,
                double* __restrict__ grad_tau, DenseArgs a) {
   using T = typename Storage<STORE>::T;
@@ -253,6 +253,7 @@ struct lbm_adjoint {
   double J = 0.0;
   int step_bx = 256;        // transposed-step workgroup width
   size_t step_lds = 0;      // its dynamic LDS floor (caps workgroups per CU)
+  int step_waves = 1;       // waves per SIMD the transposed step is compiled for (1: no bound)
 };
 
 namespace {
@@ -287,10 +288,19 @@ struct objdiff_u {
 };
 template <int Q, int S>
 struct step_u {
+  template <bool MASK, typename... A>
+  static void launch_w(dim3 g, dim3 b, hipStream_t st, size_t lds, int waves, A... a) {
+    switch (waves) {
+      case 2: hipLaunchKernelGGL((k_adj_step<Q, S, MASK, 2>), g, b, lds, st, a...); break;
+      case 3: hipLaunchKernelGGL((k_adj_step<Q, S, MASK, 3>), g, b, lds, st, a...); break;
+      case 4: hipLaunchKernelGGL((k_adj_step<Q, S, MASK, 4>), g, b, lds, st, a...); break;
+      default: hipLaunchKernelGGL((k_adj_step<Q, S, MASK, 1>), g, b, lds, st, a...); break;
+    }
+  }
   template <typename... A>
-  static void launch(dim3 g, dim3 b, hipStream_t st, size_t lds, bool mask, A... a) {
-    if (mask) hipLaunchKernelGGL((k_adj_step<Q, S, true>), g, b, lds, st, a...);
-    else hipLaunchKernelGGL((k_adj_step<Q, S, false>), g, b, lds, st, a...);
+  static void launch(dim3 g, dim3 b, hipStream_t st, size_t lds, bool mask, int waves, A... a) {
+    if (mask) launch_w<true>(g, b, st, lds, waves, a...);
+    else launch_w<false>(g, b, st, lds, waves, a...);
   }
 };
 
@@ -390,7 +400,7 @@ lbm_status backward_range(lbm_adjoint* h, const void* base, int64_t len, size_t 
     const bool mask = d->links != nullptr;
     for (int64_t j = len - 1; j >= 0; j--) {
       const void* state = j == 0 ? base : (char*)h->seg + (j - 1) * h->state_bytes;
-      launch_qs<step_u>(d, g, b, h->step_lds, mask, state, (const void*)h->lam[cur],
+      launch_qs<step_u>(d, g, b, h->step_lds, mask, h->step_waves, state, (const void*)h->lam[cur],
                         h->lam[1 - cur], h->grad[0], h->grad[1], d->args);
       LBM_LAUNCH_CHECK();
       cur = 1 - cur;
@@ -490,6 +500,14 @@ lbm_status lbm_adjoint_create(const lbm_adjoint_config* cfg, lbm_adjoint** out) 
     h->step_bx = bx;
     const char* k = std::getenv("LBM_ADJ_BLOCKS_PER_CU");
     h->step_lds = lds_for_blocks_per_cu(k ? std::atoi(k) : 0);
+    // Waves per SIMD: unbounded, the D3Q19 FP64 step needs 256 VGPRs + 6 AGPRs
+    // (one wave per SIMD); bounded to 2 it spills 6 and the 128^3 channel
+    // backward pass ran 154.8 instead of 169.8 ms (same gradients, bit for
+    // bit).  Bounds of 3-4 spill 94+ and ran slower; FP32 deviations ran best
+    // unbounded (74.9 vs 75.4 ms at 2); D3Q27 spills 90+ at 2.
+    // LBM_ADJ_WAVES=1..4 overrides (tuning).
+    const char* w = std::getenv("LBM_ADJ_WAVES");
+    h->step_waves = w ? std::atoi(w) : (d->q == 19 && d->storage == LBM_STORE_F64 ? 2 : 1);
   }
   h->state_bytes = (size_t)d->ncell * d->q * d->esize;
   h->lam_bytes = (size_t)d->ncell * d->q * (d->storage == LBM_STORE_F64 ? 8 : 4);
