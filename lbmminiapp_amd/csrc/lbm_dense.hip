@@ -395,21 +395,29 @@ DenseLaunchFn pick_step(const lbm_dense* h, int kind) {
 // multiple of 64, nx <= kDenseMaxBlock): every kind on fully periodic boxes,
 // and the pull gather also with solid cells and walls.  By default (staged
 // mode 1, set at create from LBM_DENSE_STAGED) they run where they measured
-// faster (512^3 steps, two handles alternating in one process): periodic
-// boxes in FP64 and FP32 deviations, and FP16 deviation rows for the AB steps
-// (D3Q19 pull 1.90 vs 2.05 ms, push 1.86 vs 2.05; D3Q27 push 2.81 vs 3.27)
-// except the D3Q27 pull (3.12 vs 3.02 for the kernel alone).  The FP16 AA
-// step stays unstaged (D3Q19 1.94 vs 1.91 ms).  Masked gathers run unstaged
-// (35% solid porous pull at 384^3: 20.4k vs 18.7k MLUPS staged; a 5% solid
-// channel: equal).  Mode 0 never stages, mode 2 stages every case covered
-// (tuning and cross-checks).
+// faster: 512^3 steps, staged vs unstaged handles alternating in one process,
+// ms per step (unmasked periodic boxes):
+//               AA (odd staged)     AB push            AB pull
+//   D3Q19 FP64  staged (7.68/8.47)  8.46 / 8.93        8.10 / 7.83 unstaged
+//   D3Q19 FP32  staged (3.64/3.79)  4.23 / 4.94        3.86 / 3.56 unstaged
+//   D3Q19 FP16  1.94 / 1.91 unst.   1.86 / 2.05        1.90 / 2.05
+//   D3Q27 FP64  10.65 / 11.35       11.65 / 12.69      11.71 / 10.77 unstaged
+//   D3Q27 FP32  4.64 / 4.90         5.96 / 6.45        5.75 / 5.29 unstaged
+//   D3Q27 FP16  unstaged (kernel alone 3.08 / 2.90)  push 2.81 / 3.27  pull unstaged (3.12 / 3.02)
+// Masked gathers run unstaged (35% solid porous pull at 384^3: 20.4k vs 18.7k
+// MLUPS staged; a 5% solid channel: equal).  Mode 0 never stages, mode 2
+// stages every case covered (tuning and cross-checks).
 DenseLaunchFn pick_staged(const lbm_dense* h, int kind) {
   const int mode = h->staged_mode;
   const bool mask = h->links != nullptr;
   if (mode == 0 || h->layout != LBM_LAYOUT_SOA || h->nx % 64 != 0 || h->nx > kDenseMaxBlock)
     return nullptr;
-  const bool f16_staged = kind == K_AB_PUSH || (kind == K_AB_PULL && h->q == 19);
-  if (mode < 2 && (mask || (h->storage == LBM_STORE_F16S && !f16_staged))) return nullptr;
+  if (mode < 2) {
+    const bool f16 = h->storage == LBM_STORE_F16S;
+    const bool staged = kind == K_AB_PUSH || (kind == K_AA_ODD && !f16) ||
+                        (kind == K_AB_PULL && f16 && h->q == 19);
+    if (mask || !staged) return nullptr;
+  }
   const bool force = use_force_terms(h);
   return h->q == 19 ? pick_staged19(h->storage, kind, force, h->nx, mask, h->lds_max, h->nt)
                     : pick_staged27(h->storage, kind, force, h->nx, mask, h->lds_max, h->nt);
