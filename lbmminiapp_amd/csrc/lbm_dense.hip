@@ -651,13 +651,16 @@ lbm_status dense_create(const lbm_dense_config* cfg, lbm_dense** out, bool push_
   a.djump = cfg->density_jump_x;
   // With F = 0 the Guo terms are exact zeros, so the kernels that evaluate
   // them give the same bits as the specialised ones.  For D3Q19 FP64 AA on
-  // SOA boxes without solids they also measured faster (512^3, alternating
-  // launches on one array: odd step 7.46 vs 7.98 ms, even 6.88 vs 6.94 ms);
-  // in every other case measured (FP32/FP16 deviations, D3Q27, AB) they were
-  // equal or slower.  LBM_DENSE_FORCE_TERMS=0|1 overrides.
+  // SOA boxes without solids and with rows of more than 256 cells (whole-row
+  // workgroups of more than 4 wavefronts) they also measured faster: two
+  // handles alternating in one process, MLUPS without / with the terms,
+  // 512^3: 17,436 / 18,304; 256^3: 18,437 / 18,414; 128^3: 19,083 / 18,965;
+  // 64^3: 18,373 / 16,990.  In every other case measured (FP32/FP16
+  // deviations, D3Q27, AB) they were equal or slower.  LBM_DENSE_FORCE_TERMS=0|1
+  // overrides.
   {
     const bool dflt = h->q == 19 && h->storage == LBM_STORE_F64 && h->pattern == LBM_PATTERN_AA &&
-                      h->layout == LBM_LAYOUT_SOA && h->links == nullptr;
+                      h->layout == LBM_LAYOUT_SOA && h->links == nullptr && h->nx > 256;
     const char* e = std::getenv("LBM_DENSE_FORCE_TERMS");
     h->force_terms = e ? e[0] == '1' : dflt;
   }

@@ -257,8 +257,8 @@ def test_dense_pressure_jump_f32_and_mass():
 def test_dense_zero_force_kernels(q, pattern, storage, terms, monkeypatch):
     """F = 0 runs kernels without the Guo terms (LBM_DENSE_FORCE_TERMS=0) or
     with them evaluated as exact zeros (=1, the default for D3Q19 FP64 AA on
-    open SOA boxes); for FP64 both equal the oracle (which evaluates every Guo
-    term) bit for bit."""
+    open SOA boxes with rows over 256 cells); for FP64 both equal the oracle
+    (which evaluates every Guo term) bit for bit."""
     monkeypatch.setenv("LBM_DENSE_FORCE_TERMS", terms)
     nx, ny, nz = 66, 10, 8
     geom = O.DenseGeometry(nx, ny, nz)
@@ -277,15 +277,16 @@ def test_dense_zero_force_kernels(q, pattern, storage, terms, monkeypatch):
 
 
 @pytest.mark.parametrize("terms", ["0", "1", None])
-def test_dense_staged_aa_f64_force_terms_policy(terms, monkeypatch):
-    """D3Q19 FP64 AA on a periodic SOA box with rows of 128 (the staged odd
-    step, the default-policy case): at F = 0 the kernels with and without the
-    Guo terms, and the default choice, all equal the oracle bit for bit."""
+@pytest.mark.parametrize("nx,ny,nz", [(128, 6, 5), (512, 3, 2)])
+def test_dense_staged_aa_f64_force_terms_policy(terms, nx, ny, nz, monkeypatch):
+    """D3Q19 FP64 AA on periodic SOA boxes with rows of 128 and 512 (the
+    staged odd step; with 512-cell rows the default runs the kernels with the
+    Guo terms): at F = 0 both kernel sets, and the default choice, equal the
+    oracle bit for bit."""
     if terms is not None:
         monkeypatch.setenv("LBM_DENSE_FORCE_TERMS", terms)
     else:
         monkeypatch.delenv("LBM_DENSE_FORCE_TERMS", raising=False)
-    nx, ny, nz = 128, 6, 5
     geom = O.DenseGeometry(nx, ny, nz)
     fc = seeded_fc(geom, 19, seed=19)
     omega, steps = 1 / 0.6, 6
