@@ -115,6 +115,13 @@ lbm_status lbm_sparse_ab(lbm_sparse* h, const void* src, void* tgt, int64_t star
 lbm_status lbm_sparse_init_rest(lbm_sparse* h, void* dstrb, int32_t aa);
 /* Stream-order point: waits for every launch issued on the handle. */
 lbm_status lbm_sparse_synchronize(lbm_sparse* h);
+/* Enqueue later launches on `stream` (a hipStream_t; NULL, the default, is the
+ * null stream, ordered with blocking copies as the reference's kernel calls are
+ * with its host code).  No counterpart in the reference; for callers that run
+ * the kernels on their own stream or capture their time loop as a HIP graph
+ * (the step calls only launch kernels).  Waits for the work already enqueued on
+ * the previous stream. */
+lbm_status lbm_sparse_set_stream(lbm_sparse* h, void* stream);
 
 /* ---------------------------------------------------------------- dense (fast path) */
 typedef struct lbm_dense lbm_dense;

@@ -27,7 +27,7 @@ EXPORTED = [
     "lbm_last_error", "lbm_version", "lbm_device_count", "lbm_set_device", "lbm_device_alloc",
     "lbm_device_free", "lbm_memcpy_h2d", "lbm_memcpy_d2h", "lbm_device_synchronize",
     "lbm_sparse_create", "lbm_sparse_destroy", "lbm_sparse_aa_even", "lbm_sparse_aa_odd",
-    "lbm_sparse_ab", "lbm_sparse_init_rest", "lbm_sparse_synchronize",
+    "lbm_sparse_ab", "lbm_sparse_init_rest", "lbm_sparse_synchronize", "lbm_sparse_set_stream",
     "lbm_dense_create", "lbm_dense_destroy", "lbm_dense_get_info", "lbm_dense_set_state",
     "lbm_dense_get_state", "lbm_dense_init_equilibrium", "lbm_dense_init_tgv", "lbm_dense_step",
     "lbm_dense_run_timed", "lbm_dense_moments", "lbm_dense_mass", "lbm_dense_synchronize",
@@ -103,7 +103,7 @@ def load(path: str = LIB_PATH):
         "lbm_sparse_create": [P, P], "lbm_sparse_destroy": [P],
         "lbm_sparse_aa_even": [P, P, I64, I64], "lbm_sparse_aa_odd": [P, P, I64, I64],
         "lbm_sparse_ab": [P, P, P, I64, I64, I32], "lbm_sparse_init_rest": [P, P, I32],
-        "lbm_sparse_synchronize": [P],
+        "lbm_sparse_synchronize": [P], "lbm_sparse_set_stream": [P, P],
         "lbm_dense_create": [P, P], "lbm_dense_destroy": [P], "lbm_dense_get_info": [P, P],
         "lbm_dense_set_state": [P, P], "lbm_dense_get_state": [P, P],
         "lbm_dense_init_equilibrium": [P, P, P], "lbm_dense_init_tgv": [P, D, D, ctypes.c_uint64],
@@ -256,6 +256,13 @@ class SparseKernels:
 
     def synchronize(self):
         check(load().lbm_sparse_synchronize(self.h))
+
+    def set_stream(self, stream=None):
+        """Launch on `stream` (a hipStream_t as an int, created by the HIP
+        runtime this library is bound to; None: the null stream).  Buffer
+        copies stay blocking on the null stream: synchronize() before reading
+        results that were computed on another stream."""
+        check(load().lbm_sparse_set_stream(self.h, ctypes.c_void_p(stream or None)))
 
     def close(self):
         if self.h:

@@ -9,9 +9,11 @@
 // so FP64 results and -D_PRECISION_SINGLE_ FP32 results equal the reference's
 // bit for bit.
 //
-// Launches go to the null stream so that they are ordered with the caller's
-// blocking hipMemcpy traffic, as the reference's kernel calls are ordered with
-// its host code.
+// Launches go to the null stream by default, so that they are ordered with the
+// caller's blocking hipMemcpy traffic, as the reference's kernel calls are
+// ordered with its host code; lbm_sparse_set_stream moves them to a caller's
+// stream (which can then be captured into a HIP graph: the step calls only
+// launch kernels).
 #include <cstdlib>
 
 #include "lbm_collide.hpp"
@@ -138,6 +140,7 @@ struct lbm_sparse {
   unsigned even_bx = 256, step_bx = 256;
   size_t even_lds = 0, step_lds = 0;
   bool nt = true;  // non-temporal node-aligned accesses (arrays beyond the memory-side cache)
+  hipStream_t stream = nullptr;  // null stream unless the caller set one
 };
 
 namespace {
@@ -157,13 +160,13 @@ struct Shape {
   template <int Q, typename T, int L>                                                      \
   struct name##_fn {                                                                       \
     template <typename... A>                                                               \
-    static void launch(Shape sh, bool nt, A... a) {                                        \
+    static void launch(Shape sh, bool nt, hipStream_t st, A... a) {                        \
       if (nt)                                                                              \
         hipLaunchKernelGGL((lbm::name<Q, T, L, true>), dim3(sh.grid), dim3(sh.block),      \
-                           sh.lds, 0, a...);                                               \
+                           sh.lds, st, a...);                                              \
       else                                                                                 \
         hipLaunchKernelGGL((lbm::name<Q, T, L, false>), dim3(sh.grid), dim3(sh.block),     \
-                           sh.lds, 0, a...);                                               \
+                           sh.lds, st, a...);                                              \
     }                                                                                      \
   };
 LBM_SPARSE_KERNEL_FN(k_sparse_aa_even)
@@ -176,11 +179,11 @@ LBM_SPARSE_KERNEL_FN(k_sparse_init_rest)
 template <template <int, typename, int> class K, typename T, typename... A>
 void launch_qt(const lbm_sparse* h, Shape sh, A... a) {
   if (h->q == 19) {
-    if (h->layout == LBM_LAYOUT_SOA) K<19, T, LBM_LAYOUT_SOA>::launch(sh, h->nt, a...);
-    else K<19, T, LBM_LAYOUT_AOS>::launch(sh, h->nt, a...);
+    if (h->layout == LBM_LAYOUT_SOA) K<19, T, LBM_LAYOUT_SOA>::launch(sh, h->nt, h->stream, a...);
+    else K<19, T, LBM_LAYOUT_AOS>::launch(sh, h->nt, h->stream, a...);
   } else {
-    if (h->layout == LBM_LAYOUT_SOA) K<27, T, LBM_LAYOUT_SOA>::launch(sh, h->nt, a...);
-    else K<27, T, LBM_LAYOUT_AOS>::launch(sh, h->nt, a...);
+    if (h->layout == LBM_LAYOUT_SOA) K<27, T, LBM_LAYOUT_SOA>::launch(sh, h->nt, h->stream, a...);
+    else K<27, T, LBM_LAYOUT_AOS>::launch(sh, h->nt, h->stream, a...);
   }
 }
 
@@ -361,7 +364,16 @@ lbm_status lbm_sparse_init_rest(lbm_sparse* h, void* dstrb, int32_t aa) {
 
 lbm_status lbm_sparse_synchronize(lbm_sparse* h) {
   LBM_REQUIRE(h, LBM_ERR_ARG, "lbm_sparse_synchronize: null handle");
-  LBM_HIP_CHECK(hipStreamSynchronize(0));
+  LBM_HIP_CHECK(hipStreamSynchronize(h->stream));
+  return LBM_OK;
+}
+
+// Later launches go to `stream` (NULL: the null stream).  Waits for the work
+// already enqueued on the previous stream, so the order of calls is kept.
+lbm_status lbm_sparse_set_stream(lbm_sparse* h, void* stream) {
+  LBM_REQUIRE(h, LBM_ERR_ARG, "lbm_sparse_set_stream: null handle");
+  LBM_HIP_CHECK(hipStreamSynchronize(h->stream));
+  h->stream = (hipStream_t)stream;
   return LBM_OK;
 }
 
