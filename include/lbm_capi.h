@@ -197,7 +197,7 @@ lbm_status lbm_dense_synchronize(lbm_dense* h);
  * times on the handle's current stream and advances time_step by
  * nlaunch * nsteps; it returns LBM_ERR_ARG when the handle is an odd number of
  * steps away from where it was at capture.  A graph belongs to its handle:
- * destroy the graph first. */
+ * lbm_dense_destroy returns LBM_ERR_STATE while graphs of the handle are alive. */
 typedef struct lbm_dense_graph lbm_dense_graph;
 lbm_status lbm_dense_set_stream(lbm_dense* h, void* stream);
 lbm_status lbm_dense_graph_create(lbm_dense* h, int32_t nsteps, lbm_dense_graph** out);

@@ -689,6 +689,9 @@ lbm_status lbm_dense_create(const lbm_dense_config* cfg, lbm_dense** out) {
 
 lbm_status lbm_dense_destroy(lbm_dense* h) {
   if (!h) return LBM_OK;
+  LBM_REQUIRE(h->live_graphs == 0, LBM_ERR_STATE,
+              "lbm_dense_destroy: %d graph(s) of this handle are still alive; destroy them first",
+              h->live_graphs);
   (void)hipStreamSynchronize(h->stream);
   free_all(h);
   return LBM_OK;
@@ -816,6 +819,7 @@ struct lbm_dense_graph {
   int phase = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  bool counted = false;  // included in h->live_graphs
 };
 
 static int dense_phase(const lbm_dense* h) {
@@ -826,6 +830,7 @@ lbm_status lbm_dense_graph_destroy(lbm_dense_graph* g) {
   if (!g) return LBM_OK;
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
+  if (g->counted) g->h->live_graphs--;
   delete g;
   return LBM_OK;
 }
@@ -871,6 +876,8 @@ lbm_status lbm_dense_graph_create(lbm_dense* h, int32_t nsteps, lbm_dense_graph*
     lbm_dense_graph_destroy(g);
     return st != LBM_OK ? st : LBM_ERR_HIP;
   }
+  g->counted = true;
+  h->live_graphs++;
   *out = g;
   return LBM_OK;
 }

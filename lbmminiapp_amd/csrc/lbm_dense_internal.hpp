@@ -26,6 +26,7 @@ struct lbm_dense {
   int npartial = 1024;
   hipStream_t stream = nullptr;      // the stream every call enqueues on
   hipStream_t own_stream = nullptr;  // the handle's own stream (stream unless the caller set one)
+  int live_graphs = 0;               // graphs captured from this handle and not yet destroyed
   hipEvent_t e0 = nullptr, e1 = nullptr;
   lbm::DenseArgs args{};
   size_t device_bytes = 0;

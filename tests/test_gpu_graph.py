@@ -148,6 +148,18 @@ def test_graph_rejects_odd_step_counts():
                 lat.graph(n)
 
 
+def test_handle_outlives_its_graphs():
+    """lbm_dense_destroy refuses a handle whose graphs are alive (they hold its
+    device pointers), and succeeds once they are destroyed."""
+    lat = _lattice(CASES["aa_f64_staged"])
+    g = lat.graph(2)
+    with pytest.raises(capi.LbmError):
+        lat.close()
+    g.launch(1)
+    g.close()
+    lat.close()
+
+
 @pytest.mark.parametrize("pattern", ["aa", "pull"])
 def test_sparse_graph_replays_equal_oracle(pattern):
     """The drop-in kernels on a caller's stream: two time steps of the
