@@ -14,6 +14,7 @@
 // ordered with its host code; lbm_sparse_set_stream moves them to a caller's
 // stream (which can then be captured into a HIP graph: the step calls only
 // launch kernels).
+#include <algorithm>
 #include <cstdlib>
 
 #include "lbm_collide.hpp"
@@ -43,12 +44,26 @@ __device__ __forceinline__ void stn(T* p, T v) {
   else *p = v;
 }
 
+// Node block of this workgroup.  Workgroups are dispatched to the 8 XCDs in
+// turn (block b to XCD b mod 8), each with its own L2.  group > 0 (the launch
+// then has a multiple of 8 * group blocks) gives each XCD runs of `group`
+// consecutive node blocks, so that cache lines shared by neighbouring blocks
+// (accesses through the adjacency) mostly meet in one L2: physical block p,
+// round r = p / 8, XCD k = p mod 8, takes node block
+// (r / group) 8 group + k group + r mod group, a bijection; blocks past the
+// range exit through the count check.
+__device__ __forceinline__ int64_t node_block(uint32_t group) {
+  if (!group) return blockIdx.x;
+  const uint32_t r = blockIdx.x / 8u, k = blockIdx.x % 8u;
+  return (int64_t)(r / group) * 8 * group + (int64_t)k * group + r % group;
+}
+
 // AA even: src/kernels/KernelAA_SOA_Base.cpp:173-254 (AOS: KernelAA_AOS_Base.cpp:174-254)
 template <int Q, typename T, int LAYOUT, bool NT>
 __global__ void __launch_bounds__(256)
     k_sparse_aa_even(T* __restrict__ f, int64_t n, int64_t start, int64_t count, double omega,
-                     T F0, T F1, T F2) {
-  const int64_t i = start + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+                     T F0, T F1, T F2, uint32_t group) {
+  const int64_t i = start + node_block(group) * blockDim.x + threadIdx.x;
   if (i >= start + count) return;
   T d[Q];
 #pragma unroll
@@ -63,8 +78,8 @@ __global__ void __launch_bounds__(256)
 template <int Q, typename T, int LAYOUT, bool NT>
 __global__ void __launch_bounds__(256)
     k_sparse_aa_odd(T* __restrict__ f, const int32_t* __restrict__ adj, int64_t n, int64_t start,
-                    int64_t count, double omega, T F0, T F1, T F2) {
-  const int64_t i = start + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+                    int64_t count, double omega, T F0, T F1, T F2, uint32_t group) {
+  const int64_t i = start + node_block(group) * blockDim.x + threadIdx.x;
   if (i >= start + count) return;
   T d[Q];
   int32_t loc[Q];
@@ -84,8 +99,8 @@ template <int Q, typename T, int LAYOUT, bool NT>
 __global__ void __launch_bounds__(256)
     k_sparse_ab_push(const T* __restrict__ src, T* __restrict__ tgt,
                      const int32_t* __restrict__ adj, int64_t n, int64_t start, int64_t count,
-                     double omega, T F0, T F1, T F2) {
-  const int64_t i = start + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+                     double omega, T F0, T F1, T F2, uint32_t group) {
+  const int64_t i = start + node_block(group) * blockDim.x + threadIdx.x;
   if (i >= start + count) return;
   T d[Q];
 #pragma unroll
@@ -101,8 +116,8 @@ template <int Q, typename T, int LAYOUT, bool NT>
 __global__ void __launch_bounds__(256)
     k_sparse_ab_pull(const T* __restrict__ src, T* __restrict__ tgt,
                      const int32_t* __restrict__ adj, int64_t n, int64_t start, int64_t count,
-                     double omega, T F0, T F1, T F2) {
-  const int64_t i = start + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+                     double omega, T F0, T F1, T F2, uint32_t group) {
+  const int64_t i = start + node_block(group) * blockDim.x + threadIdx.x;
   if (i >= start + count) return;
   T d[Q];
 #pragma unroll
@@ -141,6 +156,7 @@ struct lbm_sparse {
   size_t even_lds = 0, step_lds = 0;
   bool nt = true;  // non-temporal node-aligned accesses (arrays beyond the memory-side cache)
   hipStream_t stream = nullptr;  // null stream unless the caller set one
+  int xcd_env = -1;              // LBM_SPARSE_XCD: node blocks per XCD run (-1: by footprint)
 };
 
 namespace {
@@ -154,6 +170,7 @@ inline unsigned grid_for(int64_t count, unsigned bx = 256) {
 struct Shape {
   unsigned grid, block;
   size_t lds;
+  uint32_t group;  // node_block: 0, or node blocks per XCD run (grid a multiple of 8 group)
 };
 
 #define LBM_SPARSE_KERNEL_FN(name)                                                         \
@@ -187,11 +204,32 @@ void launch_qt(const lbm_sparse* h, Shape sh, A... a) {
   }
 }
 
-Shape even_shape(const lbm_sparse* h, int64_t count) {
-  return {grid_for(count, h->even_bx), h->even_bx, h->even_lds};
+// Node blocks per XCD run (node_block).  Measured on the reference's decks
+// (drop-in driver, MLUPS, none -> chosen):
+//   arrays in the memory-side cache (simulationSize 4, 175 MB): one run per
+//   XCD, AA 15.2k -> 16.2k, AB 12.8k -> 14.0k, pull 11.6k -> 11.9k;
+//   streaming (simulationSize 8, 1.35 GB): runs of 8-32 blocks raise AB push
+//   10.6k -> 11.2k, while AA and pull lost with long runs (14.7k -> 13.3k,
+//   13.4k -> 11.4k with one run per XCD) and gained nothing with short ones.
+// LBM_SPARSE_XCD (read at create) overrides: 0 off, g > 0 runs of g blocks.
+uint32_t xcd_group(const lbm_sparse* h, bool push, unsigned nb) {
+  if (h->xcd_env >= 0) return (uint32_t)h->xcd_env;
+  if (!h->nt) return (nb + 7) / 8;
+  return push ? 16u : 0u;
 }
-Shape step_shape(const lbm_sparse* h, int64_t count) {
-  return {grid_for(count, h->step_bx), h->step_bx, h->step_lds};
+
+Shape make_shape(const lbm_sparse* h, int64_t count, unsigned bx, size_t lds, bool push) {
+  const unsigned nb = grid_for(count, bx);
+  const uint32_t g = xcd_group(h, push, nb);
+  if (!g || nb < 16) return {nb, bx, lds, 0};
+  const unsigned span = 8 * g;
+  return {(nb + span - 1) / span * span, bx, lds, g};
+}
+Shape even_shape(const lbm_sparse* h, int64_t count) {
+  return make_shape(h, count, h->even_bx, h->even_lds, false);
+}
+Shape step_shape(const lbm_sparse* h, int64_t count, bool push = false) {
+  return make_shape(h, count, h->step_bx, h->step_lds, push);
 }
 
 // Launch shapes, fixed at create.  The AA even step is all node-aligned, like
@@ -259,6 +297,10 @@ lbm_status lbm_sparse_create(const lbm_sparse_config* cfg, lbm_sparse** out) {
   h->omega = cfg->omega;
   for (int d = 0; d < 3; d++) h->force[d] = cfg->force[d];
   set_shapes(h);
+  {
+    const char* e = std::getenv("LBM_SPARSE_XCD");
+    h->xcd_env = e ? std::max(0, std::min(4096, std::atoi(e))) : -1;
+  }
   if (hipMalloc(&h->adj, sizeof(int32_t) * nq) != hipSuccess) {
     delete h;
     set_error("lbm_sparse_create: hipMalloc of the adjacency failed");
@@ -289,11 +331,11 @@ lbm_status lbm_sparse_aa_even(lbm_sparse* h, void* dstrb, int64_t start, int64_t
   const Shape g = even_shape(h, count);
   if (h->storage == LBM_STORE_F64)
     launch_qt<k_sparse_aa_even_fn, double>(h, g, (double*)dstrb, h->n, start, count, h->omega,
-                                          h->force[0], h->force[1], h->force[2]);
+                                          h->force[0], h->force[1], h->force[2], g.group);
   else
     launch_qt<k_sparse_aa_even_fn, float>(h, g, (float*)dstrb, h->n, start, count, h->omega,
                                          (float)h->force[0], (float)h->force[1],
-                                         (float)h->force[2]);
+                                         (float)h->force[2], g.group);
   LBM_LAUNCH_CHECK();
   return LBM_OK;
 }
@@ -308,11 +350,11 @@ lbm_status lbm_sparse_aa_odd(lbm_sparse* h, void* dstrb, int64_t start, int64_t 
   if (h->storage == LBM_STORE_F64)
     launch_qt<k_sparse_aa_odd_fn, double>(h, g, (double*)dstrb, (const int32_t*)h->adj, h->n,
                                          start, count, h->omega, h->force[0], h->force[1],
-                                         h->force[2]);
+                                         h->force[2], g.group);
   else
     launch_qt<k_sparse_aa_odd_fn, float>(h, g, (float*)dstrb, (const int32_t*)h->adj, h->n, start,
                                         count, h->omega, (float)h->force[0], (float)h->force[1],
-                                        (float)h->force[2]);
+                                        (float)h->force[2], g.group);
   LBM_LAUNCH_CHECK();
   return LBM_OK;
 }
@@ -327,25 +369,25 @@ lbm_status lbm_sparse_ab(lbm_sparse* h, const void* src, void* tgt, int64_t star
               "lbm_sparse_ab: the reference has no AOS pull kernel");
   if (lbm_status st = check_range(h, start, count)) return st;
   if (count == 0) return LBM_OK;
-  const Shape g = step_shape(h, count);
+  const Shape g = step_shape(h, count, !pull);
   const int32_t* adj = h->adj;
   if (h->storage == LBM_STORE_F64) {
     if (pull)
       launch_qt<k_sparse_ab_pull_fn, double>(h, g, (const double*)src, (double*)tgt, adj, h->n,
                                             start, count, h->omega, h->force[0], h->force[1],
-                                            h->force[2]);
+                                            h->force[2], g.group);
     else
       launch_qt<k_sparse_ab_push_fn, double>(h, g, (const double*)src, (double*)tgt, adj, h->n,
                                             start, count, h->omega, h->force[0], h->force[1],
-                                            h->force[2]);
+                                            h->force[2], g.group);
   } else {
     const float F0 = (float)h->force[0], F1 = (float)h->force[1], F2 = (float)h->force[2];
     if (pull)
       launch_qt<k_sparse_ab_pull_fn, float>(h, g, (const float*)src, (float*)tgt, adj, h->n,
-                                           start, count, h->omega, F0, F1, F2);
+                                           start, count, h->omega, F0, F1, F2, g.group);
     else
       launch_qt<k_sparse_ab_push_fn, float>(h, g, (const float*)src, (float*)tgt, adj, h->n,
-                                           start, count, h->omega, F0, F1, F2);
+                                           start, count, h->omega, F0, F1, F2, g.group);
   }
   LBM_LAUNCH_CHECK();
   return LBM_OK;
