@@ -319,6 +319,8 @@ lbm_status lbm_sparse_create(const lbm_sparse_config* cfg, lbm_sparse** out) {
 
 lbm_status lbm_sparse_destroy(lbm_sparse* h) {
   if (!h) return LBM_OK;
+  // kernels still queued on the handle's stream read the adjacency
+  (void)hipStreamSynchronize(h->stream);
   if (h->adj) LBM_HIP_CHECK(hipFree(h->adj));
   delete h;
   return LBM_OK;
