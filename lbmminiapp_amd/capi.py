@@ -184,6 +184,39 @@ def smooth_random_field(nx, ny, nz, sigma, seed):
     return out
 
 
+MASK_MAGIC = b"LBMMASK1"
+
+
+def write_mask(path, solid, dims):
+    """Writes a solid mask file: the 8 bytes "LBMMASK1", int32 nx, ny, nz
+    (little-endian), then one byte per cell, x fastest (cell x + nx*(y + ny*z)),
+    0 fluid and 1 solid -- the format of the driver's `maskfile` / `writemask`."""
+    nx, ny, nz = (int(d) for d in dims)
+    solid = np.ascontiguousarray(solid).reshape(-1)
+    if solid.size != nx * ny * nz or min(nx, ny, nz) <= 0:
+        raise ValueError("mask size does not match its dimensions")
+    with open(path, "wb") as fh:
+        fh.write(MASK_MAGIC)
+        fh.write(np.asarray([nx, ny, nz], "<i4").tobytes())
+        fh.write((solid != 0).astype(np.uint8).tobytes())
+
+
+def read_mask(path):
+    """Reads a solid mask file (write_mask); returns (dims (nx, ny, nz), solid uint8[ncell])."""
+    with open(path, "rb") as fh:
+        raw = fh.read()
+    if raw[:8] != MASK_MAGIC or len(raw) < 20:
+        raise ValueError(f"{path} is not a mask file (LBMMASK1)")
+    nx, ny, nz = (int(d) for d in np.frombuffer(raw[8:20], "<i4"))
+    if min(nx, ny, nz) <= 0:
+        raise ValueError(f"{path}: bad dimensions")
+    n = nx * ny * nz
+    if len(raw) < 20 + n:
+        raise ValueError(f"{path} is truncated")
+    solid = (np.frombuffer(raw[20:20 + n], np.uint8) != 0).astype(np.uint8)
+    return (nx, ny, nz), solid
+
+
 # --------------------------------------------------------------------------- device buffers
 class DeviceBuffer:
     """A device allocation owned by the library (lbm_device_alloc)."""

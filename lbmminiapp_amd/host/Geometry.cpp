@@ -3,6 +3,8 @@
 #include "Geometry.hh"
 
 #include <algorithm>
+#include <cstring>
+#include <fstream>
 #include <iostream>
 #include <stdexcept>
 
@@ -12,6 +14,52 @@ namespace {
 void check(lbm_status st, const char* what) {
   if (st != LBM_OK)
     throw std::runtime_error(std::string(what) + ": " + lbm_last_error());
+}
+
+// Mask file: "LBMMASK1", int32 nx, ny, nz (little-endian), then one byte per
+// cell, x fastest, 0 fluid and 1 solid.
+const char kMaskMagic[8] = {'L', 'B', 'M', 'M', 'A', 'S', 'K', '1'};
+
+void read_mask_header(std::ifstream& in, const std::string& path, int32_t dims[3]) {
+  char magic[8];
+  in.read(magic, 8);
+  in.read(reinterpret_cast<char*>(dims), 3 * sizeof(int32_t));
+  if (!in || std::memcmp(magic, kMaskMagic, 8) != 0)
+    throw std::runtime_error("'" + path + "' is not a mask file (LBMMASK1)");
+  for (int d = 0; d < 3; d++)
+    if (dims[d] <= 0) throw std::runtime_error("mask file '" + path + "': bad dimensions");
+}
+
+void mask_file_dims(const std::string& path, int32_t dims[3]) {
+  std::ifstream in(path, std::ios::binary);
+  if (!in) throw std::runtime_error("cannot open mask file '" + path + "'");
+  read_mask_header(in, path, dims);
+}
+
+void read_mask_file(const std::string& path, const int32_t want[3], std::vector<uint8_t>& solid) {
+  std::ifstream in(path, std::ios::binary);
+  if (!in) throw std::runtime_error("cannot open mask file '" + path + "'");
+  int32_t dims[3];
+  read_mask_header(in, path, dims);
+  for (int d = 0; d < 3; d++)
+    if (dims[d] != want[d]) throw std::runtime_error("mask file '" + path + "': dimensions changed");
+  const int64_t ncell = (int64_t)dims[0] * dims[1] * dims[2];
+  solid.assign(ncell, 0);
+  in.read(reinterpret_cast<char*>(solid.data()), ncell);
+  if (in.gcount() != ncell) throw std::runtime_error("mask file '" + path + "' is truncated");
+  for (auto& v : solid) v = v ? 1 : 0;
+}
+
+void write_mask_file(const std::string& path, const int32_t dims[3],
+                     const std::vector<uint8_t>& solid) {
+  std::ofstream out(path, std::ios::binary);
+  if (!out) throw std::runtime_error("cannot write mask file '" + path + "'");
+  std::vector<uint8_t> bytes(solid.size());
+  for (size_t c = 0; c < solid.size(); c++) bytes[c] = solid[c] ? 1 : 0;
+  out.write(kMaskMagic, 8);
+  out.write(reinterpret_cast<const char*>(dims), 3 * sizeof(int32_t));
+  out.write(reinterpret_cast<const char*>(bytes.data()), (std::streamsize)bytes.size());
+  if (!out) throw std::runtime_error("writing mask file '" + path + "' failed");
 }
 }  // namespace
 
@@ -30,6 +78,11 @@ void Geometry::decomposeDomain() {
     for (int d = 0; d < 3; d++) totalLength_[d] = dims[d];
     periodic_[0] = 1;  // the reference wraps x when there is one task
     periodic_[1] = periodic_[2] = 0;
+  } else if (kind == "file") {
+    int32_t dims[3];
+    mask_file_dims(parameters_.getMaskFile(), dims);
+    for (int d = 0; d < 3; d++) totalLength_[d] = dims[d];
+    periodic_[0] = periodic_[1] = periodic_[2] = 1;  // the deck's `periodic` sets the walls
   } else {
     for (int d = 0; d < 3; d++) totalLength_[d] = parameters_.getSize(d);
     periodic_[0] = 1;
@@ -57,6 +110,9 @@ void Geometry::setupGrid() {
     check(lbm_geometry_cylinder(parameters_.getSimulationSize(), dims, solid_.data()), "cylinder");
   } else if (kind == "channel" || kind == "spheres") {
     check(lbm_geometry_channel(nx, ny, nz, solid_.data()), "channel");
+  } else if (kind == "file") {
+    const int32_t want[3] = {nx, ny, nz};
+    read_mask_file(parameters_.getMaskFile(), want, solid_);
   } else if (kind != "box" && kind != "porous") {
     throw std::runtime_error("unknown geometry '" + kind + "'");
   }
@@ -66,6 +122,10 @@ void Geometry::setupGrid() {
                                    parameters_.getRadiusMax(), parameters_.getSolidFraction(),
                                    parameters_.getSeed(), solid_.data(), &count),
           "spheres");
+  }
+  if (!parameters_.getWriteMask().empty()) {
+    const int32_t dims[3] = {nx, ny, nz};
+    write_mask_file(parameters_.getWriteMask(), dims, solid_);
   }
   grid_.assign(ncell, LAT_PT_WALL);
   fluidPts_.clear();

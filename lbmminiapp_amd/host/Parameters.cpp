@@ -66,8 +66,13 @@ void Parameters::readInput(std::string inputFile) {
     else if (k == "u0") u0_ = std::atof(num(1).c_str());
     else if (k == "noise") noise_ = std::atof(num(1).c_str());
     else if (k == "dump") dump_ = num(1);
+    else if (k == "maskfile") maskFile_ = num(1);
+    else if (k == "writemask") writeMask_ = num(1);
     else std::cerr << "warning: unknown deck key '" << k << "' ignored" << std::endl;
   }
+  if (geometry_ == "file" && maskFile_.empty())
+    throw std::runtime_error("geometry 'file' needs the deck key 'maskfile'");
+  if (!maskFile_.empty() && geometry_ != "file") geometry_ = "file";
   if (latticeSize_ != 19 && latticeSize_ != 27)
     throw std::runtime_error("deck key 'lattice' must be 19 or 27");
   if (!(omega_ > 0.0 && omega_ < 2.0)) throw std::runtime_error("omega must be in (0, 2)");

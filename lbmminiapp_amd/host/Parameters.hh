@@ -10,10 +10,14 @@
 // SOA/AOS, _PRECISION_SINGLE_; src/Parameters.hh:6-42) are runtime keys:
 //   lattice 19|27        tau T | omega W        force fx fy fz
 //   layout soa|aos        engine sparse|dense    storage f64|f32s|f16s (dense)
-//   geometry cylinder|box|channel|spheres|porous   size nx ny nz   periodic px py pz
+//   geometry cylinder|box|channel|spheres|porous|file   size nx ny nz   periodic px py pz
 //   solidfraction f  radiusmin r  radiusmax r  seed s  init rest|tgv  u0 u  noise a
 //   densityjump d (engine dense: inlet/outlet pressure drop d / 3 on the periodic x axis)
 //   dump path            (binary dump of the final populations, for checks)
+//   maskfile path        (solid mask from a mask file: geometry file, dimensions from the file)
+//   writemask path       (write the solid mask the run uses to a mask file)
+// Mask file (little-endian): the 8 bytes "LBMMASK1", int32 nx, ny, nz, then
+// nx*ny*nz bytes, cell x + nx*(y + ny*z), 0 fluid and 1 solid.
 #ifndef LBM_HOST_PARAMETERS_H
 #define LBM_HOST_PARAMETERS_H
 
@@ -61,6 +65,8 @@ class Parameters {
   double getU0() const { return u0_; }
   double getNoise() const { return noise_; }
   const std::string& getDumpPath() const { return dump_; }
+  const std::string& getMaskFile() const { return maskFile_; }
+  const std::string& getWriteMask() const { return writeMask_; }
 
  private:
   double densityChange_, simulationSize_;
@@ -72,6 +78,7 @@ class Parameters {
   double force_[3] = {1e-6, 0.0, 0.0};
   int layout_ = 0;
   std::string engine_ = "sparse", storage_ = "f64", geometry_ = "cylinder", init_ = "rest", dump_;
+  std::string maskFile_, writeMask_;
   int size_[3] = {64, 64, 64};
   int periodic_[3] = {1, 0, 0};
   bool periodicGiven_ = false;

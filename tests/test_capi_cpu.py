@@ -5,6 +5,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 from lbmminiapp_amd import capi
@@ -93,3 +94,34 @@ def test_adjoint_create_rejects_bad_checkpoint_plans(lib):
         cfg = capi._AdjointConfig(**{**base, **bad})
         assert lib.lbm_adjoint_create(ctypes.byref(cfg), ctypes.byref(h)) == capi.ERR_ARG, bad
         assert lib.lbm_last_error()
+
+
+def test_mask_file_round_trip_and_errors(tmp_path):
+    """Solid mask files (the driver's `maskfile` / `writemask` format): written
+    and read back equal, nonzero bytes read as solid, and malformed files
+    (wrong magic, bad dimensions, truncated data) are rejected."""
+    from lbmminiapp_amd import capi
+
+    dims = (7, 5, 3)
+    solid = (np.arange(7 * 5 * 3) % 4 == 0).astype(np.uint8) * 3
+    path = tmp_path / "m.bin"
+    capi.write_mask(path, solid, dims)
+    raw = path.read_bytes()
+    assert raw[:8] == b"LBMMASK1" and len(raw) == 20 + 105
+    assert list(np.frombuffer(raw[8:20], "<i4")) == [7, 5, 3]
+    got_dims, got = capi.read_mask(path)
+    assert got_dims == dims and np.array_equal(got, (solid != 0).astype(np.uint8))
+    # layout: cell x + nx*(y + ny*z)
+    assert got[2 + 7 * (1 + 5 * 2)] == (solid[2 + 7 * (1 + 5 * 2)] != 0)
+    bad = tmp_path / "bad.bin"
+    bad.write_bytes(b"LBMMASK2" + raw[8:])
+    with pytest.raises(ValueError):
+        capi.read_mask(bad)
+    bad.write_bytes(raw[:8] + np.asarray([7, 0, 3], "<i4").tobytes() + raw[20:])
+    with pytest.raises(ValueError):
+        capi.read_mask(bad)
+    bad.write_bytes(raw[:-1])
+    with pytest.raises(ValueError):
+        capi.read_mask(bad)
+    with pytest.raises(ValueError):
+        capi.write_mask(bad, solid[:-1], dims)
