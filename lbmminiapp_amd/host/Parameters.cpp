@@ -73,6 +73,11 @@ void Parameters::readInput(std::string inputFile) {
   if (geometry_ == "file" && maskFile_.empty())
     throw std::runtime_error("geometry 'file' needs the deck key 'maskfile'");
   if (!maskFile_.empty() && geometry_ != "file") geometry_ = "file";
+  // The reference's decks name a third pattern (MR = 2) that its Main.cpp
+  // never constructs (src/Main.cpp:86-121 leave the propagation unset).
+  if (propPattern_ != 0 && propPattern_ != 1)
+    throw std::runtime_error("propPattern must be 0 (AB) or 1 (AA); pattern 2 (MR) has no "
+                             "implementation");
   if (latticeSize_ != 19 && latticeSize_ != 27)
     throw std::runtime_error("deck key 'lattice' must be 19 or 27");
   if (!(omega_ > 0.0 && omega_ < 2.0)) throw std::runtime_error("omega must be in (0, 2)");

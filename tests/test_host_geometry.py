@@ -92,3 +92,57 @@ def test_deck_errors(driver, tmp_path):
     r = subprocess.run([driver, str(warn), "--dump-geometry", str(tmp_path / "g.bin")],
                        capture_output=True, text=True)
     assert r.returncode == 0 and "unknown deck key 'frobnicate'" in r.stderr
+
+
+def test_deck_rejects_unimplemented_pattern(driver, tmp_path):
+    """propPattern 2 (MR) is named in the reference decks' comments but has no
+    implementation (its Main.cpp leaves the propagation unset): an error here."""
+    deck = tmp_path / "mr.txt"
+    deck.write_text("simulationSize 0.5\npropPattern 2\n")
+    r = subprocess.run([driver, str(deck), "--dump-geometry", str(tmp_path / "g.bin")],
+                       capture_output=True, text=True)
+    assert r.returncode == 1 and "propPattern" in r.stderr
+
+
+def test_mask_file_geometry(driver, tmp_path):
+    """`maskfile`: the fluid points are exactly the mask's fluid cells, in the
+    reference's order (border first, then bulk, x-major within each); the
+    dimensions come from the file; `writemask` writes the mask back unchanged;
+    a file with another magic is an error."""
+    from lbmminiapp_amd import capi
+
+    nx, ny, nz = 12, 9, 7
+    cell = np.arange(nx * ny * nz)
+    x, y, z = cell % nx, (cell // nx) % ny, cell // (nx * ny)
+    solid = (((x - 6) ** 2 + (y - 4) ** 2 < 6) | (z == 3) & (x < 4)).astype(np.uint8)
+    mask = tmp_path / "m.mask"
+    capi.write_mask(mask, solid, (nx, ny, nz))
+    back = tmp_path / "back.mask"
+    deck = tmp_path / "file.txt"
+    deck.write_text(f"maskfile {mask}\nwritemask {back}\nperiodic 1 0 1\npropPattern 1\n")
+    out = tmp_path / "geom.bin"
+    r = subprocess.run([driver, str(deck), "--dump-geometry", str(out)], capture_output=True,
+                       text=True)
+    assert r.returncode == 0, r.stderr
+    g = read_geometry_dump(out)
+    assert tuple(int(d) for d in g["dims"]) == (nx, ny, nz)
+    fluid = set(np.nonzero(solid == 0)[0].tolist())
+    got = g["pts"][:, 0] + nx * (g["pts"][:, 1] + ny * g["pts"][:, 2])
+    assert g["n"] == len(fluid) and set(got.tolist()) == fluid
+    assert int(g["ranges"][0]) == 0  # border start; ranges = border start, count, bulk start, count
+    nb = int(g["ranges"][1])
+    pts = g["pts"]
+    on_face = ((pts[:, 0] == 0) | (pts[:, 0] == nx - 1) | (pts[:, 1] == 0) | (pts[:, 1] == ny - 1)
+               | (pts[:, 2] == 0) | (pts[:, 2] == nz - 1))
+    assert on_face[:nb].all() and not on_face[nb:].any()
+    for part in (pts[:nb], pts[nb:]):
+        key = part[:, 0] * ny * nz + part[:, 1] * nz + part[:, 2]
+        assert np.all(np.diff(key) > 0)
+    back_dims, back_solid = capi.read_mask(back)
+    assert back_dims == (nx, ny, nz) and np.array_equal(back_solid, solid)
+    bad = tmp_path / "bad.mask"
+    bad.write_bytes(b"NOTAMASK" + mask.read_bytes()[8:])
+    deck.write_text(f"maskfile {bad}\n")
+    r = subprocess.run([driver, str(deck), "--dump-geometry", str(out)], capture_output=True,
+                       text=True)
+    assert r.returncode == 1 and "not a mask file" in r.stderr
