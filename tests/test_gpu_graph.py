@@ -23,6 +23,8 @@ CASES = {
     "ab_push_f64_masked": dict(pattern="ab", pull=False, storage="f64", solid=True, force=(0, 1e-5, 0)),
     "ab_push_as_gather_f64": dict(pattern="ab", pull=False, storage="f64", solid=True, env="1"),
     "aa_aos_f64_jump": dict(pattern="aa", storage="f64", layout="aos", solid=True, jump=2e-4),
+    "q27_aa_f16s": dict(q=27, pattern="aa", storage="f16s"),
+    "q27_ab_pull_f64_masked": dict(q=27, pattern="ab", pull=True, storage="f64", solid=True),
 }
 
 
@@ -31,7 +33,7 @@ def _lattice(case):
     if case.get("solid"):
         solid = np.zeros(NX * NY * NZ, np.uint8)
         capi.add_spheres(solid, NX, NY, NZ, 2.0, 4.0, 0.15, 3, periodic=(1, 1, 1))
-    lat = DenseLattice(NX, NY, NZ, q=19, pattern=case["pattern"], pull=case.get("pull", False),
+    lat = DenseLattice(NX, NY, NZ, q=case.get("q", 19), pattern=case["pattern"], pull=case.get("pull", False),
                        layout=case.get("layout", "soa"), storage=case["storage"], omega=1 / 0.7,
                        force=case.get("force", (0.0, 0.0, 0.0)), solid=solid,
                        density_jump_x=case.get("jump", 0.0))
