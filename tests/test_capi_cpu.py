@@ -125,3 +125,41 @@ def test_mask_file_round_trip_and_errors(tmp_path):
         capi.read_mask(bad)
     with pytest.raises(ValueError):
         capi.write_mask(bad, solid[:-1], dims)
+
+
+def test_header_is_plain_c(tmp_path):
+    """include/lbm_capi.h compiles as strict C99 (the ABI is for C, cgo and
+    ctypes callers), and a C program links against the library and calls it:
+    the version string, and an argument error reported through
+    lbm_last_error without any device call."""
+    import shutil
+    import subprocess
+
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("no C compiler")
+    src = tmp_path / "use_capi.c"
+    src.write_text(r'''
+#include <stdio.h>
+#include <string.h>
+#include "lbm_capi.h"
+int main(void) {
+  lbm_dense* h = NULL;
+  lbm_status st;
+  if (!lbm_version() || !lbm_version()[0]) return 2;
+  st = lbm_dense_create(NULL, &h);
+  if (st != LBM_ERR_ARG || h != NULL || strlen(lbm_last_error()) == 0) return 3;
+  printf("%s\n", lbm_version());
+  return 0;
+}
+''')
+    exe = tmp_path / "use_capi"
+    lib_dir = os.path.join(ROOT, "lbmminiapp_amd")
+    r = subprocess.run([gcc, "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror",
+                        "-I" + os.path.join(ROOT, "include"), str(src), "-o", str(exe),
+                        "-L" + lib_dir, "-llbm_hip", "-Wl,-rpath," + lib_dir],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, (r.returncode, r.stdout, r.stderr)
+    assert r.stdout.strip()
