@@ -165,3 +165,35 @@ def test_sparse_xcd_block_mapping_vs_oracle(xcd, monkeypatch):
     want = O.run_ab(q, O.SOA, n, f0, adj, 3, omega, pull=True)
     got = run_gpu(q, "soa", "f64", omega, adj, f0, 3, "ab", pull=True)
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("prec,layout", [("f32", "soa"), ("f64", "aos"), ("f32", "aos")])
+def test_sparse_xcd_block_mapping_other_storage(prec, layout, monkeypatch):
+    """The XCD node-block runs (LBM_SPARSE_XCD=5, a run length that does not
+    divide the 240 blocks) with FP32 storage and the AOS layout: AA and AB
+    push equal the oracle bit for bit (AOS has no pull kernel in the
+    reference)."""
+    monkeypatch.setenv("LBM_SPARSE_XCD", "5")
+    nx, ny, nz, q = 48, 40, 32, 19
+    solid = (O.uniform01(35, np.arange(nx * ny * nz, dtype=np.uint64)) < 0.05).astype(np.uint8)
+    g = O.DenseGeometry(nx, ny, nz, solid, periodic=(1, 0, 1), order="reference")
+    n = g.nfluid
+    cell = g.cell_of_fluid.astype(np.uint64)
+    rho = 1.0 + 0.02 * (O.uniform01(8, 4 * cell) - 0.5)
+    u = np.stack([0.05 * (2 * O.uniform01(8, 4 * cell + np.uint64(k + 1)) - 1) for k in range(3)], 1)
+    fc = O.equilibrium(q, rho, u)
+    L = O.SOA if layout == "soa" else O.AOS
+    dt = np.float64 if prec == "f64" else np.float32
+    omega = 1 / 0.7
+    print(f"[sparse] xcd runs {prec} {layout} n={n}", flush=True)
+    adj = g.adjacency(q, L, O.ADJ_PULL)
+    f0 = O.to_storage(fc, q, L, aa_even_ready=True).astype(dt)
+    want = f0.copy()
+    O.run_aa(q, L, n, want, adj, 4, omega)
+    got = run_gpu(q, layout, prec, omega, adj, f0, 4, "aa", splits=[n // 3])
+    assert np.array_equal(got, want)
+    adjp = g.adjacency(q, L, O.ADJ_PUSH)
+    f0 = O.to_storage(fc, q, L).astype(dt)
+    want = O.run_ab(q, L, n, f0, adjp, 3, omega)
+    got = run_gpu(q, layout, prec, omega, adjp, f0, 3, "ab", adjacency_mode=0)
+    assert np.array_equal(got, want)
