@@ -18,9 +18,11 @@ stream around the same K launches.
 
 cpu_baseline: the reference's own AA kernel (KernelAA_SOA_Unroll, the
 `unroll 1` kernel of its benchmark decks) compiled from its sources into
-oracle/_ref/, one process per host core (up to 16, the reference's one MPI
-rank per core) on z-slabs of a bounded 128^3 sample of the same workload; if
-that library is absent, the C oracle (OpenMP) instead.
+oracle/_ref/, one process per host core (the reference's one MPI rank per
+core: every core of the affinity mask, within the job's CPU share) on its own
+128 x 128 x 8 z-slab of the same workload; if that library is absent, the C
+oracle (OpenMP) instead.  The line states the cores used, the affinity mask
+and nproc.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 512] [--legs f64,f16s,f32s]
 """
@@ -131,33 +133,52 @@ def adjoint_leg(n=128, steps=500, every=50, storage="f64"):
                       f"checkpoint every {every}, {storage}"}
 
 
-def _cpu_rank(rank, nranks, budget_s, barrier, out):
+SLAB_PLANES = 8  # z-planes of 128 x 128 cells per reference rank
+
+
+def _cpu_rank(rank, nranks, budget_s, barrier, probes, out):
     """One reference 'rank': the reference's KernelAA_SOA_Unroll (oracle/_ref, one thread) on its
-    z-slab of the 128^3 periodic Taylor-Green sample, timed after a common start."""
+    128 x 128 x SLAB_PLANES z-slab of the periodic Taylor-Green sample.  Every rank runs the same
+    step count (sized from the slowest rank's probe) from a common start."""
     import numpy as np
 
     from oracle import oracle as O
 
     q, n = 19, 128
-    nz = n // nranks
     omega = 1.0 / TAU
-    geom = O.DenseGeometry(n, n, nz)
+    geom = O.DenseGeometry(n, n, SLAB_PLANES)
     u = O.tgv_velocity(geom, u0=0.05, noise=1e-4, seed=1 + rank)
     fc = O.equilibrium(q, np.ones(geom.nfluid), u)
     adj = geom.adjacency(q, O.SOA, O.ADJ_PULL)
     f = O.to_storage(fc, q, O.SOA, aa_even_ready=True)
     nf = geom.nfluid
-    dt = O.ref_run("aa_soa_unroll", q, "f64", omega, nf, f.copy(), adj, 2)
-    steps = max(2, 2 * int(budget_s / dt))
+    barrier.wait()
+    probes[rank] = O.ref_run("aa_soa_unroll", q, "f64", omega, nf, f.copy(), adj, 2)
+    barrier.wait()
+    steps = max(2, 2 * int(budget_s / max(probes[:])))
     barrier.wait()
     t = O.ref_run("aa_soa_unroll", q, "f64", omega, nf, f.copy(), adj, steps)
     out.put((rank, nf, steps, t))
 
 
+def host_cores():
+    """(cores to use, cores in this process's affinity mask, nproc).  The cores used are the
+    affinity mask, capped at the host's CPU share for this job when OMP_NUM_THREADS states one
+    (a one-GPU lease allots 16 CPUs of a larger host)."""
+    nproc = os.cpu_count() or 1
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = nproc
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    use = min(affinity, share) if share > 0 else affinity
+    return max(1, use), affinity, nproc
+
+
 def cpu_baseline(budget_s):
     """The reference's own AA kernel on the host cores, one process per core as its MPI ranks
-    run (each on a z-slab of a 128^3 sample, no halo exchange, so an upper bound for the
-    reference).  Runs before any GPU work: the worker processes are spawned from a process that
+    run (each on its own 128 x 128 x SLAB_PLANES z-slab, no halo exchange, so an upper bound for
+    the reference).  Runs before any GPU work: the worker processes are spawned from a process that
     has not initialised the GPU.  Falls back to the C oracle with OpenMP when oracle/_ref is
     absent."""
     import multiprocessing as mp
@@ -165,21 +186,16 @@ def cpu_baseline(budget_s):
     from oracle import oracle as O
 
     omega = 1.0 / TAU
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    ranks = max(1, min(16, cores))
-    while 128 % ranks:
-        ranks -= 1
+    ranks, affinity, nproc = host_cores()
+    host = {"host_affinity_cores": affinity, "host_nproc": nproc}
     why = "the reference build (oracle/_ref) is absent"
     res = None
     if O.ref_available(19, "f64", omega):
         import queue
 
         ctx = mp.get_context("spawn")
-        barrier, out = ctx.Barrier(ranks), ctx.Queue()
-        procs = [ctx.Process(target=_cpu_rank, args=(r, ranks, budget_s, barrier, out))
+        barrier, out, probes = ctx.Barrier(ranks), ctx.Queue(), ctx.Array("d", ranks)
+        procs = [ctx.Process(target=_cpu_rank, args=(r, ranks, budget_s, barrier, probes, out))
                  for r in range(ranks)]
         for p in procs:
             p.start()
@@ -202,13 +218,14 @@ def cpu_baseline(budget_s):
         wall = max(t for *_, t in res)
         one = res[0]
         return {"value": lups / wall / 1e6, "unit": "MLUPS", "cores": ranks, "kind": "reference",
-                "sample": f"D3Q19 AA SOA FP64 periodic Taylor-Green 128^3 split into {ranks} z-slabs "
-                          f"of 128x128x{128 // ranks}, one process per slab and core (the "
-                          f"reference's one-rank-per-core model, without its halo exchange), each "
-                          f"running the reference's KernelAA_SOA_Unroll (unroll 1) compiled from "
-                          f"its sources with g++ -O2, omega = 1/{TAU}, ~{budget_s:.0f} s; "
-                          f"one process alone: {one[1] * one[2] / one[3] / 1e6:.2f} MLUPS "
-                          f"while all ran"}
+                **host,
+                "sample": f"D3Q19 AA SOA FP64 periodic Taylor-Green 128x128x{SLAB_PLANES * ranks} "
+                          f"split into {ranks} z-slabs of 128x128x{SLAB_PLANES}, one process per "
+                          f"slab and core (the reference's one-rank-per-core model, without its "
+                          f"halo exchange), each running the reference's KernelAA_SOA_Unroll "
+                          f"(unroll 1) compiled from its sources with g++ -O2, omega = 1/{TAU}, "
+                          f"~{budget_s:.0f} s; one process: "
+                          f"{one[1] * one[2] / one[3] / 1e6:.2f} MLUPS while all ran"}
     import numpy as np
 
     q, n = 19, 128
@@ -228,7 +245,7 @@ def cpu_baseline(budget_s):
     t0 = time.perf_counter()
     O.run_aa(q, O.SOA, nf, g, adj, steps, omega, (0.0, 0.0, 0.0))
     dt = time.perf_counter() - t0
-    return {"value": nf * steps / dt / 1e6, "unit": "MLUPS", "cores": ranks, "kind": "port",
+    return {"value": nf * steps / dt / 1e6, "unit": "MLUPS", "cores": ranks, "kind": "port", **host,
             "sample": f"D3Q19 AA SOA FP64 periodic Taylor-Green box {n}^3 ({nf} nodes) x {steps} "
                       f"steps, C oracle (restatement of KernelAA_SOA_Base) with OpenMP, "
                       f"{ranks} threads; {why}"}

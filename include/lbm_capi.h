@@ -113,6 +113,16 @@ lbm_status lbm_sparse_ab(lbm_sparse* h, const void* src, void* tgt, int64_t star
 /* PropagationAA::initialize (aa = 1, w_s into slot opp(s)) / PropagationAB::initialize
  * (aa = 0, w_s into slot s) (src/propagation/PropagationAA.cpp:60-123, PropagationAB.cpp:91-152). */
 lbm_status lbm_sparse_init_rest(lbm_sparse* h, void* dstrb, int32_t aa);
+/* FP64 mass of the fluid-index range [start, start + count) of a population
+ * array: the sum of the Q stored values of each node, accumulated in FP64 by a
+ * deterministic two-pass reduction (over the whole range, the total mass in
+ * either slot convention).  The reference has no mass reduction; it replaces
+ * summing PropagationAA/AB::getRho over the host copy
+ * (src/propagation/PropagationAA.cpp:218-257).  Allocates a partial-sum
+ * workspace of ceil(count / 4096) + 1 doubles per call, waits for the result;
+ * LBM_ERR_NUMERIC if it is not finite. */
+lbm_status lbm_sparse_mass(lbm_sparse* h, const void* dstrb, int64_t start, int64_t count,
+                           double* mass);
 /* Stream-order point: waits for every launch issued on the handle. */
 lbm_status lbm_sparse_synchronize(lbm_sparse* h);
 /* Enqueue later launches on `stream` (a hipStream_t; NULL, the default, is the

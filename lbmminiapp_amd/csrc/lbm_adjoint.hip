@@ -59,12 +59,18 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, 256), amdgpu_waves
   const int y = blockIdx.y, z = blockIdx.z;
   const int64_t cell = x + (int64_t)a.nx * (y + (int64_t)a.ny * z);
   uint32_t lk = 0;
+  const int64_t n = a.ncell;
   if constexpr (MASK) {
     lk = a.links[cell];
-    if (lk & SOLID_BIT) return;
+    if (lk & SOLID_BIT) {
+      // a solid cell has no populations: its adjoint slots are zero in both
+      // ping-pong arrays, so lambda^0 reads back as 0 there
+#pragma unroll
+      for (int k = 0; k < Q; k++) lam_out[slot<Q, LBM_LAYOUT_SOA>(n, k, cell)] = R(0);
+      return;
+    }
   }
   const Offsets off(x, y, z, a.nx, a.ny, a.nz);
-  const int64_t n = a.ncell;
   double fs[Q];
   R lg[Q];
   double rho = 0.0, m0 = 0.0, m1 = 0.0, m2 = 0.0;
@@ -125,11 +131,19 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, 256), amdgpu_waves
   if (grad_tau) grad_tau[cell] += -om * om * (double)dw;
 }
 
+// Tree sum over the workgroup, for any blockDim.x <= 256 (the seed and
+// difference kernels run blocks of 64-256, e.g. 192 for rows of 129-192).
+// The halving starts at the power of two p with p/2 < blockDim.x <= p; a lane
+// adds its partner only when the partner exists, so every entry is summed once.
 __device__ __forceinline__ double block_sum(double v, double* red) {
+  const int nb = (int)blockDim.x;
   red[threadIdx.x] = v;
   __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+  int p = 1;
+  while (p < nb) p <<= 1;
+  for (int w = p >> 1; w > 0; w >>= 1) {
+    const int t = (int)threadIdx.x;
+    if (t < w && t + w < nb) red[t] += red[t + w];
     __syncthreads();
   }
   return red[0];

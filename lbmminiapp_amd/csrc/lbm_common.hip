@@ -6,7 +6,12 @@ namespace lbm {
 
 static thread_local std::string g_last_error;
 
+// Records the message of a failed call.  It also resets HIP's per-thread
+// last-error state, which a failed runtime call such as an out-of-memory
+// hipMalloc leaves set: otherwise the next launch check of any handle would
+// report that old failure.
 void set_error(const char* fmt, ...) {
+  (void)hipGetLastError();
   char buf[1024];
   va_list ap;
   va_start(ap, fmt);

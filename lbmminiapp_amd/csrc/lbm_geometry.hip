@@ -8,6 +8,7 @@
 // the counter-based generator the rest of the engine uses.
 #include <algorithm>
 #include <cmath>
+#include <thread>
 #include <vector>
 
 #include "lbm_internal.hpp"
@@ -139,24 +140,45 @@ lbm_status lbm_field_smooth_random(int32_t nx, int32_t ny, int32_t nz, double si
     wsum += wts[i + radius];
   }
   for (double& w : wts) w /= wsum;
-  const int dims[3] = {nx, ny, nz};
-  const int64_t strides[3] = {1, nx, (int64_t)nx * ny};
-  for (int d = 0; d < 3; d++) {
-    const int n = dims[d];
-    const int64_t st = strides[d];
-    for (int z = 0; z < nz; z++)
-      for (int y = 0; y < ny; y++)
-        for (int x = 0; x < nx; x++) {
-          const int p[3] = {x, y, z};
-          const int64_t c = cell_of(x, y, z, nx, ny);
-          const int64_t base = c - (int64_t)p[d] * st;
-          double acc = 0.0;
-          for (int i = -radius; i <= radius; i++) {
-            const int q = ((p[d] + i) % n + n) % n;
-            acc += wts[i + radius] * a[base + (int64_t)q * st];
+  // One pass per dimension.  Each output x-row is accumulated tap by tap
+  // (i = -radius .. radius, acc = 0 + w_i a_i + ...), so every cell sees the
+  // same sum in the same order whichever dimension is filtered; the y and z
+  // passes read whole source rows (contiguous), and the planes are split over
+  // host threads (the result does not depend on the split).
+  auto pass = [&](int d, int z0, int z1) {
+    std::vector<double> acc(nx);
+    for (int z = z0; z < z1; z++)
+      for (int y = 0; y < ny; y++) {
+        double* dst = &b[cell_of(0, y, z, nx, ny)];
+        if (d == 0) {
+          const double* row = &a[cell_of(0, y, z, nx, ny)];
+          for (int x = 0; x < nx; x++) {
+            double s = 0.0;
+            for (int i = -radius; i <= radius; i++)
+              s += wts[i + radius] * row[((x + i) % nx + nx) % nx];
+            dst[x] = s;
           }
-          b[c] = acc;
+          continue;
         }
+        std::fill(acc.begin(), acc.end(), 0.0);
+        for (int i = -radius; i <= radius; i++) {
+          const int yy = d == 1 ? ((y + i) % ny + ny) % ny : y;
+          const int zz = d == 2 ? ((z + i) % nz + nz) % nz : z;
+          const double* row = &a[cell_of(0, yy, zz, nx, ny)];
+          const double w = wts[i + radius];
+          for (int x = 0; x < nx; x++) acc[x] += w * row[x];
+        }
+        std::copy(acc.begin(), acc.end(), dst);
+      }
+  };
+  const unsigned hw = std::thread::hardware_concurrency();
+  const int nthreads = std::max(1, std::min<int>(nz, std::min<unsigned>(hw ? hw : 1, 16)));
+  for (int d = 0; d < 3; d++) {
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nthreads; t++)
+      pool.emplace_back(pass, d, (int)((int64_t)nz * t / nthreads),
+                        (int)((int64_t)nz * (t + 1) / nthreads));
+    for (auto& th : pool) th.join();
     a.swap(b);
   }
   const auto mm = std::minmax_element(a.begin(), a.end());

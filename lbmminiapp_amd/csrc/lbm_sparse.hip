@@ -15,6 +15,7 @@
 // stream (which can then be captured into a HIP graph: the step calls only
 // launch kernels).
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 
 #include "lbm_collide.hpp"
@@ -140,6 +141,52 @@ __global__ void __launch_bounds__(256) k_sparse_init_rest(T* __restrict__ f, int
   }
 }
 
+// FP64 mass of a fluid-index range: the sum of the Q stored values of every
+// node in [start, start + count) (over the whole range this is the total mass
+// in any slot convention, which only permutes the populations over the
+// slots).  Deterministic two-pass reduction with a fixed shape: workgroup b of
+// 256 lanes sums the kMassChunk nodes start + b kMassChunk + ... (lane t takes
+// nodes t, t + 256, ... of the chunk, each node's values in s order, in FP64)
+// into partial[b]; one workgroup then sums the partials in a fixed order.
+constexpr int kMassBlock = 256;
+constexpr int64_t kMassChunk = 16 * kMassBlock;
+
+__device__ __forceinline__ double mass_block_sum(double v) {
+  __shared__ double red[kMassBlock];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = kMassBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  return red[0];
+}
+
+// (NT is unused: the launcher instantiates every kernel for both cache policies.)
+template <int Q, typename T, int LAYOUT, bool NT>
+__global__ void __launch_bounds__(kMassBlock)
+    k_sparse_mass_partial(const T* __restrict__ f, int64_t n, int64_t start, int64_t count,
+                          double* __restrict__ partial) {
+  const int64_t c0 = (int64_t)blockIdx.x * kMassChunk;
+  double acc = 0.0;
+  for (int64_t k = threadIdx.x; k < kMassChunk; k += kMassBlock) {
+    if (c0 + k >= count) break;
+    const int64_t i = start + c0 + k;
+#pragma unroll
+    for (int s = 0; s < Q; s++) acc += (double)f[pidx<Q, LAYOUT>(n, s, i)];
+  }
+  const double tot = mass_block_sum(acc);
+  if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(kMassBlock)
+    k_sparse_mass_final(const double* __restrict__ partial, int64_t nb, double* __restrict__ out) {
+  double acc = 0.0;
+  for (int64_t b = threadIdx.x; b < nb; b += kMassBlock) acc += partial[b];
+  const double tot = mass_block_sum(acc);
+  if (threadIdx.x == 0) *out = tot;
+}
+
 }  // namespace lbm
 
 struct lbm_sparse {
@@ -191,6 +238,7 @@ LBM_SPARSE_KERNEL_FN(k_sparse_aa_odd)
 LBM_SPARSE_KERNEL_FN(k_sparse_ab_push)
 LBM_SPARSE_KERNEL_FN(k_sparse_ab_pull)
 LBM_SPARSE_KERNEL_FN(k_sparse_init_rest)
+LBM_SPARSE_KERNEL_FN(k_sparse_mass_partial)
 #undef LBM_SPARSE_KERNEL_FN
 
 template <template <int, typename, int> class K, typename T, typename... A>
@@ -266,6 +314,29 @@ lbm_status check_range(const lbm_sparse* h, int64_t start, int64_t count) {
               (long long)(start + count), (long long)h->n);
   return LBM_OK;
 }
+
+// The two mass-reduction launches of a range into `partial` (nb entries) and
+// `out`, on the handle's stream.  Shared by the allocating and the
+// allocation-free entry points, so both give the same bits.
+lbm_status launch_mass(const lbm_sparse* h, const void* dstrb, int64_t start, int64_t count,
+                       double* partial, int64_t nb, double* out) {
+  if (nb > 0) {
+    const Shape g{(unsigned)nb, (unsigned)kMassBlock, 0, 0};
+    if (h->storage == LBM_STORE_F64)
+      launch_qt<k_sparse_mass_partial_fn, double>(h, g, (const double*)dstrb, h->n, start, count,
+                                                  partial);
+    else
+      launch_qt<k_sparse_mass_partial_fn, float>(h, g, (const float*)dstrb, h->n, start, count,
+                                                 partial);
+    LBM_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(k_sparse_mass_final, dim3(1), dim3(kMassBlock), 0, h->stream,
+                     (const double*)partial, nb, out);
+  LBM_LAUNCH_CHECK();
+  return LBM_OK;
+}
+
+int64_t mass_blocks(int64_t count) { return (count + kMassChunk - 1) / kMassChunk; }
 
 }  // namespace
 
@@ -403,6 +474,29 @@ lbm_status lbm_sparse_init_rest(lbm_sparse* h, void* dstrb, int32_t aa) {
   else
     launch_qt<k_sparse_init_rest_fn, float>(h, g, (float*)dstrb, h->n, (int)aa);
   LBM_LAUNCH_CHECK();
+  return LBM_OK;
+}
+
+lbm_status lbm_sparse_mass(lbm_sparse* h, const void* dstrb, int64_t start, int64_t count,
+                           double* mass) {
+  LBM_REQUIRE(h && dstrb && mass, LBM_ERR_ARG, "lbm_sparse_mass: null pointer");
+  if (lbm_status st = check_range(h, start, count)) return st;
+  const int64_t nb = mass_blocks(count);
+  double* partial = nullptr;
+  LBM_HIP_CHECK(hipMalloc(&partial, sizeof(double) * (nb + 1)));
+  lbm_status st = launch_mass(h, dstrb, start, count, partial, nb, partial + nb);
+  hipError_t e = hipSuccess;
+  if (st == LBM_OK) e = hipMemcpyAsync(mass, partial + nb, sizeof(double), hipMemcpyDeviceToHost,
+                                       h->stream);
+  if (st == LBM_OK && e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  (void)hipFree(partial);
+  if (st != LBM_OK) return st;
+  if (e != hipSuccess) {
+    set_error("lbm_sparse_mass: %s", hipGetErrorString(e));
+    return LBM_ERR_HIP;
+  }
+  LBM_REQUIRE(std::isfinite(*mass), LBM_ERR_NUMERIC, "lbm_sparse_mass: mass is not finite (%g)",
+              *mass);
   return LBM_OK;
 }
 

@@ -207,7 +207,8 @@ double orc_objective(int q, int objective, int64_t n, const double* f,
 int orc_adjoint(int q, int64_t n, const int32_t* adj, const double* f0, int nsteps,
                 double omega, const double* F, const double* tau_node, const double* gamma_node,
                 int objective, const int32_t* cell_of_fluid, int nx, int outlet_x,
-                double* f_final, double* j_out, double* grad_gamma, double* grad_tau) {
+                double* f_final, double* j_out, double* grad_gamma, double* grad_tau,
+                double* lam0) {
   int32_t dirs[81], opp[27];
   double w[27];
   if (orc_lattice(q, dirs, opp, w)) return -1;
@@ -265,6 +266,7 @@ int orc_adjoint(int q, int64_t n, const int32_t* adj, const double* f0, int nste
     lam = lam2;
     lam2 = tmp;
   }
+  if (lam0) memcpy(lam0, lam, sizeof(double) * nq);
   free(states);
   free(omega_node);
   free(lam);

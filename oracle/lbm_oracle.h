@@ -92,7 +92,8 @@ double orc_mass(int64_t n, const double* f);
 /* ---- discrete adjoint of the canonical push map f^{n+1} = S(C(f^n)) (double) ----
  * objective 0: J = sum_x 0.5 |m|^2 / rho   (kinetic energy of the final state)
  * objective 1: J = sum_{x: x_coord == outlet_x} m_x                (outlet flux)
- * Gradients are written per fluid node for gamma (Brinkman) and tau (= 1/omega).
+ * Gradients are written per fluid node for gamma (Brinkman) and tau (= 1/omega);
+ * lam0 (nullable, q * nfluid, canonical order) receives lambda^0 = dJ/df^0.
  */
 double orc_objective(int q, int objective, int64_t nfluid, const double* f_canon,
                      const int32_t* cell_of_fluid, int nx, int outlet_x);
@@ -100,7 +101,7 @@ int orc_adjoint(int q, int64_t nfluid, const int32_t* adj_push, const double* f0
                 int nsteps, double omega, const double* force, const double* tau_node,
                 const double* gamma_node, int objective, const int32_t* cell_of_fluid,
                 int nx, int outlet_x, double* f_final, double* j_out, double* grad_gamma,
-                double* grad_tau);
+                double* grad_tau, double* lam0);
 
 #ifdef __cplusplus
 }

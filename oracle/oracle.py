@@ -64,9 +64,11 @@ def lib():
         L.orc_lattice.argtypes = [_I, _P, _P, _P]
         L.orc_build_adjacency.argtypes = [_I, _I, _I, _I, _I, _I, _P, _P, _P, _I64, _P]
         L.orc_equilibrium.argtypes = [_I, _D, _P, _P]
+        L.orc_moments.argtypes = [_I, _I64, _P, _P, _P]
         L.orc_objective.argtypes = [_I, _I, _I64, _P, _P, _I, _I]
         L.orc_objective.restype = _D
-        L.orc_adjoint.argtypes = [_I, _I64, _P, _P, _I, _D, _P, _P, _P, _I, _P, _I, _I, _P, _P, _P, _P]
+        L.orc_adjoint.argtypes = [_I, _I64, _P, _P, _I, _D, _P, _P, _P, _I, _P, _I, _I, _P, _P, _P, _P,
+                                  _P]
         _lib = L
     return _lib
 
@@ -385,8 +387,9 @@ def run_unroll19(kind, n, f, adj, steps, omega, t0=0):
 
 
 def adjoint(q, geom: DenseGeometry, f0_canon, nsteps, omega, force, tau_node=None,
-            gamma_node=None, objective=0, outlet_x=None):
-    """FP64 adjoint of the canonical push map.  Returns (J, grad_gamma, grad_tau, f_final)."""
+            gamma_node=None, objective=0, outlet_x=None, with_lambda0=False):
+    """FP64 adjoint of the canonical push map.  Returns (J, grad_gamma, grad_tau, f_final),
+    plus lambda^0 = dJ/df^0 ([q, nfluid]) when with_lambda0."""
     L = lib()
     n = geom.nfluid
     adj = geom.adjacency(q, SOA, ADJ_PUSH)
@@ -398,13 +401,28 @@ def adjoint(q, geom: DenseGeometry, f0_canon, nsteps, omega, force, tau_node=Non
     gt = np.zeros(n)
     ff = np.zeros(n * q)
     J = np.zeros(1)
+    lam0 = np.zeros(n * q) if with_lambda0 else None
     ox = geom.nx - 1 if outlet_x is None else outlet_x
     rc = L.orc_adjoint(q, n, _ptr(adj), _ptr(f0), nsteps, omega, _ptr(F), _ptr(tn), _ptr(gn),
                        objective, _ptr(geom.cell_of_fluid), geom.nx, ox, _ptr(ff), _ptr(J),
-                       _ptr(gg), _ptr(gt))
+                       _ptr(gg), _ptr(gt), _ptr(lam0))
     if rc != 0:
         raise RuntimeError(f"orc_adjoint failed: {rc}")
+    if with_lambda0:
+        return float(J[0]), gg, gt, ff.reshape(q, n), lam0.reshape(q, n)
     return float(J[0]), gg, gt, ff.reshape(q, n)
+
+
+def moments(q, f_canon):
+    """Density and velocity of each node from canonical f[q, n], in the
+    reference readout's operation order (src/propagation/PropagationAA.cpp:264-327:
+    sums over s in order, u = m * (1 / rho), no F/2 shift).  Returns rho[n], u[n, 3]."""
+    f = _f(np.asarray(f_canon, np.float64))
+    n = f.shape[1]
+    rho = np.empty(n)
+    u = np.empty((n, 3))
+    lib().orc_moments(q, n, _ptr(f), _ptr(rho), _ptr(u))
+    return rho, u
 
 
 def objective(q, geom: DenseGeometry, f_canon, kind=0, outlet_x=None):

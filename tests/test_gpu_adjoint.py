@@ -122,12 +122,18 @@ def test_config4_gradient_vs_finite_differences():
 
 
 def test_config4_fp32_adjoint_vs_fp64():
-    n, steps = 64, 200
+    """Config 4 at its full extent in FP32 deviation storage (FP32 forward and
+    adjoint arithmetic, FP64 gradient accumulation): D3Q19 128^3 channel,
+    smooth gamma field, kinetic energy after 500 steps, checkpoint every 50.
+    J within 1e-5 relative and the gamma gradient within 1e-3 rel-L2 of the
+    FP64 adjoint (itself pinned to finite differences at 1e-6)."""
+    n, steps = 128, 500
     solid, gamma = config4_case(n)
     out = {}
     for st in ("f64", "f32s"):
+        print(f"[adjoint] config4 128^3 {st}", flush=True)
         with Adjoint(n, n, n, q=19, storage=st, omega=1 / 0.8, force=(1e-6, 0, 0), solid=solid,
-                     gamma=gamma, checkpoint_every=25) as adj:
+                     gamma=gamma, checkpoint_every=50) as adj:
             J = adj.forward(steps)
             gg, _, ms = adj.backward(tau=False)
             out[st] = (J, gg)
@@ -136,8 +142,8 @@ def test_config4_fp32_adjoint_vs_fp64():
     fluid = solid == 0
     rel = np.linalg.norm(g32[fluid] - g64[fluid]) / np.linalg.norm(g64[fluid])
     print(f"[adjoint] FP32 vs FP64: J rel {abs(J32 - J64) / J64:.3e}, gradient rel-L2 {rel:.3e}")
-    assert abs(J32 - J64) / J64 < 1e-3
-    assert rel < 1e-2
+    assert abs(J32 - J64) / J64 < 1e-5
+    assert rel < 1e-3
 
 
 def test_outlet_flux_tau_gradient_vs_finite_differences():
@@ -186,3 +192,55 @@ def test_pressure_drop_adjoint_vs_finite_differences():
         for cell in picks:
             d = adj.finite_difference("tau", int(cell), 1e-2, steps, order=6)
             assert abs(gt[cell] - d) <= 1e-6 * abs(d) + 1e-12 * np.abs(gt).max(), (cell, gt[cell], d)
+
+
+@pytest.mark.parametrize("nx", [160, 100])
+def test_adjoint_non_power_of_two_rows(nx):
+    """Rows of 160 and 100 cells run the seed and difference reductions in
+    workgroups of 192 and 128 lanes (rounded up to whole wavefronts).  J, the
+    final state and the gradients equal the FP64 oracle; two seeded central
+    differences agree with the adjoint gradient."""
+    q, ny, nz = 19, 6, 4
+    solid = capi.channel(nx, ny, nz)
+    g = O.DenseGeometry(nx, ny, nz, solid, periodic=(1, 1, 1))
+    cells = np.arange(g.ncell, dtype=np.uint64)
+    gamma = 0.1 + 0.5 * O.uniform01(61, cells)
+    cell = g.cell_of_fluid.astype(np.uint64)
+    u = np.stack([0.03 * (2 * O.uniform01(62, 4 * cell + np.uint64(k + 1)) - 1) for k in range(3)], 1)
+    fc = O.equilibrium(q, np.ones(g.nfluid), u)
+    full = np.zeros((q, g.ncell))
+    full[:, g.cell_of_fluid] = fc
+    steps, F = 12, (1e-5, 0.0, 0.0)
+    print(f"[adjoint] rows of {nx}", flush=True)
+    with Adjoint(nx, ny, nz, q=q, storage="f64", omega=1 / 0.7, force=F, solid=solid,
+                 gamma=gamma, checkpoint_every=5) as adj:
+        adj.set_initial_state(full)
+        J = adj.forward(steps)
+        gg, _, _ = adj.backward(tau=False)
+        picks = g.cell_of_fluid[(O.uniform01(63, np.arange(2, dtype=np.uint64)) * g.nfluid).astype(int)]
+        fds = [adj.finite_difference("gamma", int(c), 1e-3, steps, order=6) for c in picks]
+    Jo, ggo, _, _ = O.adjoint(q, g, fc, steps, 1 / 0.7, F, gamma_node=gamma[g.cell_of_fluid])
+    assert abs(J - Jo) <= 1e-13 * abs(Jo), (J, Jo)
+    assert np.abs(gg[g.cell_of_fluid] - ggo).max() <= 1e-10 * np.abs(ggo).max()
+    for c, d in zip(picks, fds):
+        assert abs(gg[c] - d) <= 1e-6 * abs(d) + 1e-9 * np.abs(gg).max(), (c, gg[c], d)
+
+
+@pytest.mark.parametrize("steps", [10, 11])
+def test_adjoint_lambda0_vs_oracle(steps):
+    """lambda^0 = dJ/df^0 read back after even and odd step counts (the two
+    ping-pong arrays) equals the oracle's (pinned by finite differences in the
+    initial populations) in every fluid slot, and is zero in every solid cell."""
+    q = 19
+    g, solid, gamma, tau, fc, full = small_channel(q, seed=9)
+    F = (2e-5, 0.0, 0.0)
+    with Adjoint(g.nx, g.ny, g.nz, q=q, storage="f64", omega=1.0, force=F, solid=solid, tau=tau,
+                 gamma=gamma, checkpoint_every=4) as adj:
+        adj.set_initial_state(full)
+        adj.forward(steps)
+        adj.backward()
+        lam = adj.lambda0()
+    _, _, _, _, lam_o = O.adjoint(q, g, fc, steps, 1.0, F, tau_node=tau[g.cell_of_fluid],
+                                  gamma_node=gamma[g.cell_of_fluid], with_lambda0=True)
+    assert np.abs(lam[:, g.cell_of_fluid] - lam_o).max() <= 1e-12 * np.abs(lam_o).max()
+    assert not lam[:, solid != 0].any()

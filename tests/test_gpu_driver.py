@@ -183,3 +183,58 @@ def test_mask_file_reproduces_generated_geometry(engine, tmp_path):
         assert np.array_equal(again["adj"], gen["adj"])
         assert np.array_equal(again["pts"], gen["pts"])
     assert np.array_equal(again["f"], gen["f"])
+
+
+def _vtk_origin(path):
+    with open(path, "rb") as fh:
+        for _ in range(6):
+            ln = fh.readline()
+    assert ln.startswith(b"ORIGIN")
+    return tuple(int(v) for v in ln.split()[1:4])
+
+
+@pytest.mark.parametrize("pattern,steps", [("aa", 41), ("aa", 42), ("push", 41), ("push", 42),
+                                           ("pull", 41), ("pull", 42)])
+def test_readout_vtk_vs_oracle_moments(pattern, steps, tmp_path):
+    """The host readout behind writeVTK (PropagationAA::getRho/getVelocity,
+    parity-aware: slot opp(s) after even step counts, through the adjacency
+    after odd ones; PropagationAB: local slots of the current array) on the
+    cylinder deck (simulationSize 1), after odd and even step counts.  Every
+    fluid point's rho and velocity in the binary VTK equal the oracle's
+    moments (reference operation order) of the oracle's own state, bit for
+    bit; solid points hold zeros."""
+    pp = {"aa": (1, 0), "push": (0, 0), "pull": (0, 1)}[pattern]
+    deck = (f"simulationSize 1\npropPattern {pp[0]}\npull {pp[1]}\nunroll 1\n"
+            f"timesteps {steps}\nwritevtk 2\n")
+    (tmp_path / "deck.txt").write_text(deck + f"dump {tmp_path / 'd.bin'}\n")
+    print(f"[driver] readout {pattern} {steps}", flush=True)
+    r = subprocess.run([DRIVER, "deck.txt"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=240)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    d = read_dump(tmp_path / "d.bin")
+    vtk = read_vtk(tmp_path / "outs.00000.vtk")
+    x0, y0, z0 = _vtk_origin(tmp_path / "outs.00000.vtk")
+    n, q = d["n"], d["q"]
+    adj = d["adj"].copy()
+    f0 = np.zeros(q * n)
+    if pattern == "aa":
+        O.lib().orc_init_rest_aa_f64(q, O.SOA, n, O._ptr(f0))
+        want = f0.copy()
+        O.run_aa(q, O.SOA, n, want, adj, steps, 1.0, (1e-6, 0.0, 0.0))
+        canon = O.aa_canonical(want, q, n, O.SOA, adj, steps)
+    else:
+        O.lib().orc_init_rest_ab_f64(q, O.SOA, n, O._ptr(f0))
+        want = O.run_ab(q, O.SOA, n, f0, adj, steps, 1.0, pull=pattern == "pull",
+                        force=(1e-6, 0.0, 0.0))
+        canon = O.from_storage(want, q, n, O.SOA)  # local slots, as the AB readout reads them
+    assert np.array_equal(d["f"], want)
+    rho, u = O.moments(q, canon)
+    nx, ny, _ = vtk["dims"]
+    pts = d["pts"].astype(np.int64)
+    idx = (pts[:, 0] - x0) + nx * ((pts[:, 1] - y0) + ny * (pts[:, 2] - z0))
+    assert np.array_equal(vtk["id"][idx], np.arange(n))
+    assert np.array_equal(vtk["rho"][idx], rho)
+    assert np.array_equal(vtk["velocity"][idx], u)
+    solid = vtk["id"] < 0
+    assert solid.any() and not vtk["rho"][solid].any() and not vtk["velocity"][solid].any()
+    assert np.abs(u[:, 0]).max() > 0

@@ -197,3 +197,50 @@ def test_sparse_xcd_block_mapping_other_storage(prec, layout, monkeypatch):
     want = O.run_ab(q, L, n, f0, adjp, 3, omega)
     got = run_gpu(q, layout, prec, omega, adjp, f0, 3, "ab", adjacency_mode=0)
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("q,layout,prec", [(19, "soa", "f64"), (27, "aos", "f64"),
+                                           (19, "soa", "f32")])
+def test_sparse_mass_reduction(q, layout, prec):
+    """lbm_sparse_mass: the FP64 sum of the stored values of a fluid range,
+    against the exact (math.fsum) sum of the same values; bit-identical on a
+    repeated call; unchanged to round-off by AA steps (mass conservation);
+    LBM_ERR_NUMERIC on a NaN; ranges longer than one partial-sum chunk and
+    ragged ones."""
+    import math
+
+    from lbmminiapp_amd import capi
+
+    geom = O.DenseGeometry(40, 30, 20, (O.uniform01(5, np.arange(24000, dtype=np.uint64)) < 0.2)
+                           .astype(np.uint8))
+    n = geom.nfluid
+    cell = geom.cell_of_fluid.astype(np.uint64)
+    u = np.stack([0.03 * (2 * O.uniform01(6, 4 * cell + np.uint64(k)) - 1) for k in range(3)], 1)
+    fc = O.equilibrium(q, 1.0 + 0.01 * O.uniform01(7, cell), u)
+    lay = O.SOA if layout == "soa" else O.AOS
+    adj = geom.adjacency(q, lay, O.ADJ_PULL)
+    dt = np.float64 if prec == "f64" else np.float32
+    f0 = O.to_storage(fc, q, lay, aa_even_ready=True).astype(dt)
+    k = SparseKernels(n, q, adj, 1.2, layout=layout, storage=prec)
+    try:
+        buf = DeviceBuffer.from_array(f0)
+        vals = O.from_storage(f0.astype(np.float64), q, n, lay)
+        for start, count in ((0, n), (0, 4096), (1234, 9000), (n - 17, 17), (3, 0)):
+            m = k.mass(buf, start, count)
+            exact = math.fsum(vals[:, start:start + count].reshape(-1))
+            assert abs(m - exact) <= 1e-14 * max(abs(exact), 1.0), (start, count, m, exact)
+            assert k.mass(buf, start, count) == m
+        m0 = k.mass(buf)
+        for t in range(6):
+            (k.aa_even if t % 2 == 0 else k.aa_odd)(buf)
+        m1 = k.mass(buf)
+        assert abs(m1 - m0) <= (1e-13 if prec == "f64" else 1e-5) * m0
+        bad = f0.copy()
+        bad[7] = np.nan
+        buf.upload(bad)
+        with pytest.raises(capi.LbmError) as ei:
+            k.mass(buf)
+        assert ei.value.status == capi.ERR_NUMERIC
+        buf.free()
+    finally:
+        k.close()

@@ -90,3 +90,30 @@ def test_objective_closed_forms():
     assert abs(ke - g.nfluid * 0.5 * 1.1 * (0.02 ** 2 + 0.01 ** 2)) < 1e-15 * g.nfluid
     flux = O.objective(q, g, f, 1, outlet_x=3)
     assert abs(flux - (g.nfluid / 6) * 1.1 * 0.02) < 1e-15 * g.nfluid
+
+
+@pytest.mark.parametrize("objective", [0, 1])
+def test_adjoint_lambda0_matches_finite_differences(objective):
+    """lambda^0 = dJ/df^0: perturbing one initial population f_s(x) changes J by
+    lambda^0_s(x) times the perturbation (4th-order central differences), for
+    seeded (s, x) pairs including a population next to the obstacle."""
+    q = 19
+    g, f0, gamma, tau = channel_case(q, seed=4)
+    steps, omega, F = 9, 1.0, (2e-5, 0.0, 0.0)
+    J, _, _, _, lam0 = O.adjoint(q, g, f0, steps, omega, F, tau_node=tau, gamma_node=gamma,
+                                 objective=objective, outlet_x=g.nx - 2, with_lambda0=True)
+    n = g.nfluid
+    assert lam0.shape == (q, n)
+
+    def J_f(ff):
+        return O.adjoint(q, g, ff.reshape(q, n), steps, omega, F, tau_node=tau,
+                         gamma_node=gamma, objective=objective, outlet_x=g.nx - 2)[0]
+
+    flat = np.ascontiguousarray(f0.reshape(-1))
+    picks = (O.uniform01(77, np.arange(8, dtype=np.uint64)) * flat.size).astype(int)
+    scale = np.abs(lam0).max()
+    assert scale > 0
+    for i in picks:
+        d = fd(J_f, flat, i, 1e-4)
+        got = lam0.reshape(-1)[i]
+        assert abs(got - d) <= 1e-6 * max(abs(d), 1e-3 * scale), (i, got, d)

@@ -264,3 +264,31 @@ def test_poiseuille_bounce_back_channel():
     assert np.abs(ux - expect).max() / expect.max() < 1e-9
     # mass is conserved to round-off by the Base kernels with bounce-back
     assert abs(out.sum() - fc.sum()) / fc.sum() < 1e-12
+
+
+# --------------------------------------------------------------------------- readout
+@pytest.mark.parametrize("q", [19, 27])
+def test_moments_readout_closed_forms(q):
+    """orc_moments (the reference readout, u = m / rho without the F/2 shift):
+    moments of f^eq(rho, u) give back rho and u (sum f^eq = rho, sum f^eq c =
+    rho u); swapping two opposite populations flips the momentum they carry;
+    the rest population carries mass only."""
+    n = 64
+    cell = np.arange(n, dtype=np.uint64)
+    rho = 1.0 + 0.05 * (O.uniform01(5, cell) - 0.5)
+    u = np.stack([0.04 * (2 * O.uniform01(6 + k, cell) - 1) for k in range(3)], 1)
+    f = O.equilibrium(q, rho, u)
+    r, v = O.moments(q, f)
+    assert np.abs(r - rho).max() < 1e-15 and np.abs(v - u).max() < 1e-15
+    c, opp, _ = O.lattice(q)
+    g = f.copy()
+    g[[0, opp[0]]] = g[[opp[0], 0]]  # swap f_+x and f_-x
+    r2, v2 = O.moments(q, g)
+    assert np.array_equal(r2, r)
+    dx = (f[0] - f[opp[0]]) / rho
+    assert np.abs(v2[:, 0] - (v[:, 0] - 2 * dx)).max() < 1e-15
+    h = f.copy()
+    h[q - 1] += 0.25
+    r3, v3 = O.moments(q, h)
+    assert np.abs(r3 - (r + 0.25)).max() < 1e-15
+    assert np.abs(v3 * r3[:, None] - v * r[:, None]).max() < 1e-15
