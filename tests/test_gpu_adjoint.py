@@ -125,8 +125,9 @@ def test_config4_fp32_adjoint_vs_fp64():
     """Config 4 at its full extent in FP32 deviation storage (FP32 forward and
     adjoint arithmetic, FP64 gradient accumulation): D3Q19 128^3 channel,
     smooth gamma field, kinetic energy after 500 steps, checkpoint every 50.
-    J within 1e-5 relative and the gamma gradient within 1e-3 rel-L2 of the
-    FP64 adjoint (itself pinned to finite differences at 1e-6)."""
+    J within 1e-7 relative and the gamma gradient within 1e-6 rel-L2 of the
+    FP64 adjoint (itself pinned to finite differences at 1e-6); measured on
+    the MI355X: J 1.2e-8, gradient 3.4e-7."""
     n, steps = 128, 500
     solid, gamma = config4_case(n)
     out = {}
@@ -142,8 +143,8 @@ def test_config4_fp32_adjoint_vs_fp64():
     fluid = solid == 0
     rel = np.linalg.norm(g32[fluid] - g64[fluid]) / np.linalg.norm(g64[fluid])
     print(f"[adjoint] FP32 vs FP64: J rel {abs(J32 - J64) / J64:.3e}, gradient rel-L2 {rel:.3e}")
-    assert abs(J32 - J64) / J64 < 1e-5
-    assert rel < 1e-3
+    assert abs(J32 - J64) / J64 < 1e-7
+    assert rel < 1e-6
 
 
 def test_outlet_flux_tau_gradient_vs_finite_differences():
