@@ -271,6 +271,14 @@ lbm_status lbm_adjoint_get_final_state(lbm_adjoint* h, double* f);
  * truncation (h^order) from above.  Invalidates the last forward pass. */
 lbm_status lbm_adjoint_fd(lbm_adjoint* h, int32_t param, int64_t cell, double step,
                           int32_t nsteps, int32_t order, double* derivative);
+/* The same central difference along a whole-field direction v (host, ncell
+ * doubles; solid cells ignored): d/de J(p + e v) at e = 0, whose adjoint value
+ * is sum_x grad(x) v(x).  One cell's influence on J can sit below the FP64
+ * round-off of J itself on large grids; a direction over many cells keeps the
+ * difference well above it.  tau + e v must stay above 1/2. */
+lbm_status lbm_adjoint_fd_direction(lbm_adjoint* h, int32_t param, const double* direction,
+                                    double step, int32_t nsteps, int32_t order,
+                                    double* derivative);
 lbm_status lbm_adjoint_device_bytes(lbm_adjoint* h, size_t* bytes);
 
 /* ---------------------------------------------------------------- geometry and fields

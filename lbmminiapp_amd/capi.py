@@ -38,7 +38,8 @@ EXPORTED = [
     "lbm_field_smooth_random", "lbm_copy_bandwidth",
     "lbm_adjoint_create", "lbm_adjoint_destroy", "lbm_adjoint_set_initial_state",
     "lbm_adjoint_forward", "lbm_adjoint_backward", "lbm_adjoint_get_lambda0",
-    "lbm_adjoint_get_final_state", "lbm_adjoint_fd", "lbm_adjoint_device_bytes",
+    "lbm_adjoint_get_final_state", "lbm_adjoint_fd", "lbm_adjoint_fd_direction",
+    "lbm_adjoint_device_bytes",
 ]
 
 
@@ -121,6 +122,7 @@ def load(path: str = LIB_PATH):
         "lbm_adjoint_set_initial_state": [P, P], "lbm_adjoint_forward": [P, I32, P],
         "lbm_adjoint_backward": [P, P, P, P], "lbm_adjoint_get_lambda0": [P, P],
         "lbm_adjoint_get_final_state": [P, P], "lbm_adjoint_fd": [P, I32, I64, D, I32, I32, P],
+        "lbm_adjoint_fd_direction": [P, I32, P, D, I32, I32, P],
         "lbm_adjoint_device_bytes": [P, P],
     }
     for name, args in sig.items():
@@ -508,6 +510,16 @@ class Adjoint:
         p = {"gamma": 0, "tau": 1}[param] if isinstance(param, str) else int(param)
         check(load().lbm_adjoint_fd(self.h, p, int(cell), float(step), int(nsteps), int(order),
                                     ctypes.byref(d)))
+        return d.value
+
+    def finite_difference_direction(self, param, direction, step, nsteps, order=6) -> float:
+        """d/de J(p + e v) at e = 0 for the whole-field direction v (ncell values)."""
+        d = ctypes.c_double(0)
+        v = np.ascontiguousarray(direction, np.float64).reshape(-1)
+        assert v.size == self.ncell
+        p = {"gamma": 0, "tau": 1}[param] if isinstance(param, str) else int(param)
+        check(load().lbm_adjoint_fd_direction(self.h, p, _p(v), float(step), int(nsteps),
+                                              int(order), ctypes.byref(d)))
         return d.value
 
     def device_bytes(self) -> int:

@@ -264,6 +264,7 @@ struct lbm_adjoint {
   int64_t npartial = 0;
   size_t state_bytes = 0, lam_bytes = 0, extra_bytes = 0;
   std::vector<double> tau_host, gamma_host;
+  std::vector<uint8_t> solid_host;
   double J = 0.0;
   int step_bx = 256;        // transposed-step workgroup width
   size_t step_lds = 0;      // its dynamic LDS floor (caps workgroups per CU)
@@ -385,6 +386,30 @@ lbm_status ensure_buffers(lbm_adjoint* h, int64_t T) {
       return st;
     h->seg_states = need_seg;
   }
+  return LBM_OK;
+}
+
+// The whole gamma (param 0) or tau (param 1) field p0 + e v on the device (solid
+// cells keep the values create gave them); tau must stay above 1/2.
+lbm_status set_field(lbm_adjoint* h, int param, const double* v, double e) {
+  lbm_dense* d = h->d;
+  const std::vector<double>& p0 = param == 0 ? h->gamma_host : h->tau_host;
+  std::vector<double> field(d->ncell);
+  for (int64_t c = 0; c < d->ncell; c++) {
+    const double p = p0[c] + e * v[c];
+    if (param == 0) {
+      field[c] = p;
+    } else {
+      const bool solid = !h->solid_host.empty() && h->solid_host[c];
+      LBM_REQUIRE(solid || (std::isfinite(p) && p > 0.5), LBM_ERR_ARG,
+                  "lbm_adjoint_fd_direction: tau + e v = %g at cell %lld is not above 1/2", p,
+                  (long long)c);
+      field[c] = solid ? 1.0 : 1.0 / p;
+    }
+  }
+  double* dst = param == 0 ? d->gamma_field : d->omega_field;
+  LBM_HIP_CHECK(hipStreamSynchronize(d->stream));
+  LBM_HIP_CHECK(hipMemcpy(dst, field.data(), sizeof(double) * d->ncell, hipMemcpyHostToDevice));
   return LBM_OK;
 }
 
@@ -526,6 +551,7 @@ lbm_status lbm_adjoint_create(const lbm_adjoint_config* cfg, lbm_adjoint** out) 
   h->state_bytes = (size_t)d->ncell * d->q * d->esize;
   h->lam_bytes = (size_t)d->ncell * d->q * (d->storage == LBM_STORE_F64 ? 8 : 4);
   if (cfg->tau) h->tau_host.assign(cfg->tau, cfg->tau + d->ncell);
+  if (cfg->solid) h->solid_host.assign(cfg->solid, cfg->solid + d->ncell);
   if (cfg->gamma) h->gamma_host.assign(cfg->gamma, cfg->gamma + d->ncell);
   const dim3 g = adj_grid(d);
   h->npartial = (int64_t)g.x * g.y * g.z;
@@ -646,6 +672,53 @@ lbm_status lbm_adjoint_get_final_state(lbm_adjoint* h, double* f) {
   return lbm_dense_get_state(d, f);
 }
 
+}  // extern "C"
+
+namespace {
+
+// Central difference of order 2, 4 or 6 of J along `set(e)` (which puts the
+// parameter field at p0 + e v on the device): D_k = J(+kh) - J(-kh), each summed
+// node by node, then the weights of the order; `set(0)` restores the field.
+template <typename Set>
+lbm_status central_difference(lbm_adjoint* h, Set set, double step, int32_t nsteps, int32_t order,
+                              double* derivative) {
+  lbm_dense* d = h->d;
+  for (int i = 0; i < 2; i++)
+    if (!h->fd[i])
+      if (lbm_status st = dmalloc(d, &h->fd[i], h->state_bytes, "finite-difference state")) return st;
+  double dj[3] = {0.0, 0.0, 0.0};
+  const int npairs = order / 2;
+  for (int k = 0; k < npairs; k++) {
+    const double delta = (k + 1) * step;
+    for (int sgn = 0; sgn < 2; sgn++) {
+      if (lbm_status st = set(sgn == 0 ? delta : -delta)) {
+        (void)set(0.0);
+        return st;
+      }
+      void* fin = nullptr;
+      if (lbm_status st = run_forward(h, nsteps, false, &fin)) return st;
+      LBM_HIP_CHECK(hipMemcpyAsync(h->fd[sgn], fin, h->state_bytes, hipMemcpyDeviceToDevice,
+                                   d->stream));
+    }
+    launch_qs<objdiff_u>(d, adj_grid(d), adj_block(d), (const void*)h->fd[0],
+                         (const void*)h->fd[1], d->args, h->objective, h->outlet_x, h->partial);
+    LBM_LAUNCH_CHECK();
+    if (lbm_status st = reduce_to_host(h, &dj[k])) return st;
+  }
+  if (lbm_status st = set(0.0)) return st;
+  h->final_state = nullptr;  // the engine's arrays were reused
+  h->T = -1;
+  // central-difference weights of orders 2, 4 and 6
+  if (order == 2) *derivative = dj[0] / (2.0 * step);
+  else if (order == 4) *derivative = (8.0 * dj[0] - dj[1]) / (12.0 * step);
+  else *derivative = (45.0 * dj[0] - 9.0 * dj[1] + dj[2]) / (60.0 * step);
+  return LBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 lbm_status lbm_adjoint_fd(lbm_adjoint* h, int32_t param, int64_t cell, double step,
                           int32_t nsteps, int32_t order, double* derivative) {
   LBM_REQUIRE(h && derivative && nsteps >= 1 && step > 0.0, LBM_ERR_ARG,
@@ -657,36 +730,24 @@ lbm_status lbm_adjoint_fd(lbm_adjoint* h, int32_t param, int64_t cell, double st
   LBM_REQUIRE(cell >= 0 && cell < d->ncell, LBM_ERR_ARG, "lbm_adjoint_fd: cell out of range");
   LBM_REQUIRE(param == 0 ? !h->gamma_host.empty() : !h->tau_host.empty(), LBM_ERR_STATE,
               "lbm_adjoint_fd: the handle has no %s field", param == 0 ? "gamma" : "tau");
-  for (int i = 0; i < 2; i++)
-    if (!h->fd[i])
-      if (lbm_status st = dmalloc(d, &h->fd[i], h->state_bytes, "finite-difference state")) return st;
   const double p0 = param == 0 ? h->gamma_host[cell] : h->tau_host[cell];
-  // D_k = J(p + k h) - J(p - k h), k = 1 .. order / 2, each summed node by node
-  double dj[3] = {0.0, 0.0, 0.0};
-  const int npairs = order / 2;
-  for (int k = 0; k < npairs; k++) {
-    const double delta = (k + 1) * step;
-    for (int sgn = 0; sgn < 2; sgn++) {
-      const double v = sgn == 0 ? p0 + delta : p0 - delta;
-      if (lbm_status st = set_param(h, param, cell, v)) return st;
-      void* fin = nullptr;
-      if (lbm_status st = run_forward(h, nsteps, false, &fin)) return st;
-      LBM_HIP_CHECK(hipMemcpyAsync(h->fd[sgn], fin, h->state_bytes, hipMemcpyDeviceToDevice,
-                                   d->stream));
-    }
-    launch_qs<objdiff_u>(d, adj_grid(d), adj_block(d), (const void*)h->fd[0],
-                         (const void*)h->fd[1], d->args, h->objective, h->outlet_x, h->partial);
-    LBM_LAUNCH_CHECK();
-    if (lbm_status st = reduce_to_host(h, &dj[k])) return st;
-  }
-  if (lbm_status st = set_param(h, param, cell, p0)) return st;
-  h->final_state = nullptr;  // the engine's arrays were reused
-  h->T = -1;
-  // central-difference weights of orders 2, 4 and 6
-  if (order == 2) *derivative = dj[0] / (2.0 * step);
-  else if (order == 4) *derivative = (8.0 * dj[0] - dj[1]) / (12.0 * step);
-  else *derivative = (45.0 * dj[0] - 9.0 * dj[1] + dj[2]) / (60.0 * step);
-  return LBM_OK;
+  auto set = [&](double e) { return set_param(h, param, cell, p0 + e); };
+  return central_difference(h, set, step, nsteps, order, derivative);
+}
+
+lbm_status lbm_adjoint_fd_direction(lbm_adjoint* h, int32_t param, const double* direction,
+                                    double step, int32_t nsteps, int32_t order,
+                                    double* derivative) {
+  LBM_REQUIRE(h && direction && derivative && nsteps >= 1 && step > 0.0, LBM_ERR_ARG,
+              "lbm_adjoint_fd_direction: bad arguments");
+  LBM_REQUIRE(order == 2 || order == 4 || order == 6, LBM_ERR_ARG,
+              "lbm_adjoint_fd_direction: order must be 2, 4 or 6");
+  LBM_REQUIRE(param == 0 || param == 1, LBM_ERR_ARG,
+              "lbm_adjoint_fd_direction: param 0 = gamma, 1 = tau");
+  LBM_REQUIRE(param == 0 ? !h->gamma_host.empty() : !h->tau_host.empty(), LBM_ERR_STATE,
+              "lbm_adjoint_fd_direction: the handle has no %s field", param == 0 ? "gamma" : "tau");
+  auto set = [&](double e) { return set_field(h, param, direction, e); };
+  return central_difference(h, set, step, nsteps, order, derivative);
 }
 
 lbm_status lbm_adjoint_device_bytes(lbm_adjoint* h, size_t* bytes) {

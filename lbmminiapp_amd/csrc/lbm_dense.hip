@@ -10,6 +10,7 @@
 //   AB push         f_s(x) in slot s at x of the current array
 //   AB pull         the current array holds post-collision values; f_s(x) is gathered
 //                   like the AA odd case (src/kernels/KernelAB_SOA_Pull.cpp:139-140)
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <vector>
@@ -371,6 +372,23 @@ void set_launch_shapes(lbm_dense* h) {
     h->lds_max = (size_t)lds;
   h->even_bx = clamp_bx(env_int("LBM_DENSE_EVEN_BLOCK_X", even_bx));
   h->even_lds = lds_for_blocks_per_cu(env_int("LBM_DENSE_EVEN_BLOCKS_PER_CU", even_k));
+  // Segmented odd step (k_dense_seg_odd): unmasked periodic SOA boxes whose
+  // rows split into segments of a multiple of 64 cells.  512^3 D3Q19 odd step,
+  // ms (whole-row staged / best segmented shape): FP32 deviations 3.55 / 3.34
+  // (256 wide, <= 4 per CU), so FP32 rows of 512 or more take it; FP64 7.46 /
+  // 7.86 and FP16 1.99 (unstaged) / 2.74 stay as they are.  LBM_DENSE_ODD_SEG
+  // = width (0 off) and LBM_DENSE_ODD_SEG_BLOCKS_PER_CU override (tuning).  The
+  // mask is known only later in create, so dense_create clears it for masked
+  // grids.
+  {
+    const bool dflt = h->storage == LBM_STORE_F32S && h->q == 19 && h->nx >= 512 &&
+                      h->nx % 256 == 0;
+    const int w = env_int("LBM_DENSE_ODD_SEG", dflt ? 256 : 0);
+    const bool ok = h->layout == LBM_LAYOUT_SOA && h->pattern == LBM_PATTERN_AA && w >= 64 &&
+                    w <= 256 && w % 64 == 0 && h->nx % w == 0;
+    h->odd_seg_w = ok ? w : 0;
+    h->odd_seg_lds = lds_for_blocks_per_cu(env_int("LBM_DENSE_ODD_SEG_BLOCKS_PER_CU", dflt ? 4 : 0));
+  }
 }
 
 // state convention of the current array
@@ -423,24 +441,74 @@ DenseLaunchFn pick_staged(const lbm_dense* h, int kind) {
                     : pick_staged27(h->storage, kind, force, h->nx, mask, h->lds_max, h->nt);
 }
 
-lbm_status launch_kind(lbm_dense* h, int kind, void* f0, void* f1) {
+lbm_status launch_kind(lbm_dense* h, int kind, void* f0, void* f1, int z0, int nzl) {
+  if (nzl < 0) {
+    z0 = 0;
+    nzl = h->nz;
+  }
+  DenseArgs a = h->args;
+  a.z0 = z0;
+  if (kind == K_AA_ODD && h->odd_seg_w > 0) {
+    const int w = h->odd_seg_w;
+    const bool force = use_force_terms(h);
+    DenseLaunchFn fn = h->q == 19 ? pick_seg_odd19(h->storage, force, h->nt)
+                                  : pick_seg_odd27(h->storage, force, h->nt);
+    LBM_REQUIRE(fn, LBM_ERR_UNSUPPORTED, "no segmented odd kernel for this configuration");
+    fn(dim3((unsigned)((h->nx / w) * h->ny * nzl)), dim3(w), h->stream, f0, f1, a, h->odd_seg_lds);
+    LBM_LAUNCH_CHECK();
+    return LBM_OK;
+  }
   if (DenseLaunchFn st = pick_staged(h, kind)) {
-    st(dim3(1, h->ny, h->nz), dim3(h->nx), h->stream, f0, f1, h->args, h->step_lds);
+    st(dim3(1, h->ny, nzl), dim3(h->nx), h->stream, f0, f1, a, h->step_lds);
   } else {
     DenseLaunchFn fn = pick_step(h, kind);
     LBM_REQUIRE(fn, LBM_ERR_UNSUPPORTED, "no kernel for this configuration");
     const bool even = kind == K_AA_EVEN;
     const dim3 b(even ? h->even_bx : h->step_bx);
-    const dim3 g((h->nx + b.x - 1) / b.x, h->ny, h->nz);
-    fn(g, b, h->stream, f0, f1, h->args, even ? h->even_lds : h->step_lds);
+    const dim3 g((h->nx + b.x - 1) / b.x, h->ny, nzl);
+    fn(g, b, h->stream, f0, f1, a, even ? h->even_lds : h->step_lds);
   }
   LBM_LAUNCH_CHECK();
   return LBM_OK;
 }
 
+// One AA step pair (even, then odd) as z-slabs of P planes, so that the odd
+// step of a slab runs right after the even steps it reads, while their planes
+// are still in the memory-side cache.  The odd step at z reads and writes
+// slots of planes z - 1 .. z + 1, all of which the even step must have
+// updated first (and no odd step may precede the even step of a plane it
+// writes).  Order with S slabs: even(S-1), even(0), then for k = 0 .. S-2
+// even(k+1), odd(k), then odd(S-1).  Every node still gets one even and one
+// odd update with the same arithmetic, so the result equals the whole-box
+// launches bit for bit.
+lbm_status enqueue_aa_pair_by_slabs(lbm_dense* h) {
+  const int P = h->slab_planes, S = (h->nz + P - 1) / P;
+  void* f = h->f[0];
+  auto run = [&](int kind, int k) {
+    const int z0 = k * P;
+    return launch_kind(h, kind, f, f, z0, std::min(P, h->nz - z0));
+  };
+  if (lbm_status st = run(K_AA_EVEN, S - 1)) return st;
+  if (S > 1)
+    if (lbm_status st = run(K_AA_EVEN, 0)) return st;
+  for (int k = 0; k + 1 < S; k++) {
+    if (k + 1 < S - 1)
+      if (lbm_status st = run(K_AA_EVEN, k + 1)) return st;
+    if (lbm_status st = run(K_AA_ODD, k)) return st;
+  }
+  return run(K_AA_ODD, S - 1);
+}
+
 lbm_status enqueue_steps(lbm_dense* h, int nsteps) {
   for (int i = 0; i < nsteps; i++) {
     lbm_status st;
+    if (h->pattern == LBM_PATTERN_AA && h->slab_planes > 0 && h->t % 2 == 0 && i + 1 < nsteps) {
+      st = enqueue_aa_pair_by_slabs(h);
+      if (st != LBM_OK) return st;
+      h->t += 2;
+      i++;
+      continue;
+    }
     if (h->pattern == LBM_PATTERN_AA) {
       st = launch_kind(h, (h->t % 2 == 0) ? K_AA_EVEN : K_AA_ODD, h->f[0], h->f[0]);
     } else {
@@ -521,6 +589,7 @@ lbm_status dense_create(const lbm_dense_config* cfg, lbm_dense** out, bool push_
   h->storage = cfg->storage;
   for (int k = 0; k < 3; k++) h->periodic[k] = cfg->periodic[k] ? 1 : 0;
   if (const char* e = std::getenv("LBM_DENSE_STAGED")) h->staged_mode = std::atoi(e);
+  if (const char* e = std::getenv("LBM_DENSE_SLAB_PLANES")) h->slab_planes = std::max(0, std::atoi(e));
   h->ncell = (int64_t)cfg->nx * cfg->ny * cfg->nz;
   h->esize = cfg->storage == LBM_STORE_F64 ? 8 : (cfg->storage == LBM_STORE_F32S ? 4 : 2);
   set_launch_shapes(h);  // after ncell and esize: the cache policy depends on the footprint
@@ -533,6 +602,7 @@ lbm_status dense_create(const lbm_dense_config* cfg, lbm_dense** out, bool push_
   }
   const bool need_links =
       cfg->solid != nullptr || !h->periodic[0] || !h->periodic[1] || !h->periodic[2];
+  if (need_links) h->odd_seg_w = 0;  // the segmented odd step covers unmasked periodic boxes
   // AB push on a masked SOA grid runs the pull (gather) kernel one streaming
   // step out of phase: with P_0 = S^-1 f_0, (C S)^T P_0 = S^-1 (S C)^T f_0, so
   // the push state f_T is the gather of the pull storage, which is how every

@@ -15,6 +15,9 @@ struct lbm_dense {
   size_t lds_max = 65536;  // the device's LDS limit per workgroup
   bool nt = true;          // streaming (non-temporal) accesses in the time-step kernels
   bool force_terms = false;  // time-step kernels with the Guo terms also at F = 0
+  int slab_planes = 0;       // AA steps in pairs as z-slabs of this many planes (0: whole box)
+  int odd_seg_w = 0;         // AA odd step as row segments of this width (k_dense_seg_odd; 0: off)
+  size_t odd_seg_lds = 0;    // its dynamic LDS floor (caps workgroups per CU)
   int64_t ncell = 0, nfluid = 0, t = 0;
   size_t esize = 8;
   void* f[2] = {nullptr, nullptr};
@@ -37,8 +40,9 @@ namespace lbm {
 // launch shapes of the utility kernels: 256-wide x segments per (y, z) row
 dim3 block_for(int nx);
 dim3 grid_for(const lbm_dense* h);
-// one time-step launch of `kind` from f0 to f1 (f0 == f1 for AA), staged when possible
-lbm_status launch_kind(lbm_dense* h, int kind, void* f0, void* f1);
+// one time-step launch of `kind` from f0 to f1 (f0 == f1 for AA), staged when possible,
+// over the z-planes [z0, z0 + nzl) (nzl < 0: the whole box)
+lbm_status launch_kind(lbm_dense* h, int kind, void* f0, void* f1, int z0 = 0, int nzl = -1);
 // dynamic LDS that leaves room for at most k workgroups per CU (0: no cap)
 size_t lds_for_blocks_per_cu(int k);
 // device allocation accounted in h->device_bytes

@@ -51,6 +51,8 @@ struct DenseArgs {
   double F0, F1, F2;
   float scale, inv_scale;      // FP16 deviation scaling
   double djump;                // density jump across the periodic x boundary (0: none)
+  int z0;                      // first z-plane of the launch (z = blockIdx.z + z0): 0 except in
+                               // the z-slab schedule of the AA steps
 };
 
 // Pressure-drop boundary: the value a population carries across the periodic
@@ -160,6 +162,56 @@ struct Offsets {
   }
 };
 
+// SOA element (s, x', y', z') addressed as the row (s, y', z') plus the lane's
+// x'.  Within a workgroup every lane has the same (y, z), so the row address
+// is uniform.  row_elem moves it into scalar registers (readfirstlane: the
+// value is the same in every lane) as a global-memory pointer and adds the
+// lane's 32-bit byte offset, so that each access is one load or store with a
+// scalar base and a 32-bit lane offset, instead of a 64-bit per-lane address
+// sum per access (the compiler otherwise folds the 19 row bases into one
+// per-lane address chain).
+template <typename T>
+__device__ __forceinline__ T* row_of(T* f, int64_t n, int s, const DenseArgs& a, int yy, int zz) {
+  return f + (n * s + (int64_t)a.nx * (yy + (int64_t)a.ny * zz));
+}
+template <typename T>
+using global_ptr = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ global_ptr<T> row_elem(T* row, uint32_t xx) {
+  const uint64_t v = (uint64_t)row;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  const auto base = (global_ptr<const char>)(((uint64_t)hi << 32) | lo);
+  return (global_ptr<T>)(base + xx * (uint32_t)sizeof(T));
+}
+// Loads and stores of one row element (non-temporal when NT), converting FP16
+// deviations as ld / st do.
+template <int STORE, bool NT>
+__device__ __forceinline__ typename Storage<STORE>::C ld_row(const typename Storage<STORE>::T* row,
+                                                             uint32_t xx, float inv_scale) {
+  const auto p = row_elem(row, xx);
+  typename Storage<STORE>::T v;
+  if constexpr (NT) v = __builtin_nontemporal_load(p);
+  else v = *p;
+  if constexpr (STORE == LBM_STORE_F16S) return (float)v * inv_scale;
+  else return v;
+}
+template <int STORE, bool NT>
+__device__ __forceinline__ void st_row(typename Storage<STORE>::T* row, uint32_t xx,
+                                       typename Storage<STORE>::C v, float scale) {
+  using T = typename Storage<STORE>::T;
+  const auto p = row_elem(row, xx);
+  T w;
+  if constexpr (STORE == LBM_STORE_F16S) w = (T)(v * scale);
+  else w = v;
+  if constexpr (NT) __builtin_nontemporal_store(w, p);
+  else *p = w;
+}
+// periodic wrap of v + (-1, 0 or 1) into [0, nv)
+__device__ __forceinline__ int wrap1(int v, int nv) {
+  return v < 0 ? v + nv : (v >= nv ? v - nv : v);
+}
+
 // true when x + c leaves the box through a non-periodic face
 __device__ __forceinline__ bool leaves_box(const DenseArgs& a, int x, int y, int z, int cx, int cy,
                                            int cz) {
@@ -230,7 +282,7 @@ __global__ void __launch_bounds__(kDenseMaxBlock)
   using C = typename Storage<STORE>::C;
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= a.nx) return;
-  const int y = blockIdx.y, z = blockIdx.z;
+  const int y = blockIdx.y, z = blockIdx.z + a.z0;
   const int64_t cell = x + (int64_t)a.nx * (y + (int64_t)a.ny * z);
   uint32_t lk = 0;
   bool solid = false;
@@ -246,18 +298,37 @@ __global__ void __launch_bounds__(kDenseMaxBlock)
   const Offsets off(x, y, z, a.nx, a.ny, a.nz);
   const int64_t n = a.ncell;
 
+  // Fully periodic SOA boxes (no mask): every access is a row element, see row_of.
+  constexpr bool ROWS = !MASK && LAYOUT == LBM_LAYOUT_SOA;
   C d[Q];
   if (!solid) {
 #pragma unroll
     for (int s = 0; s < Q; s++) {
       const int o = Lattice<Q>::opp(s);
       const int cx = Lattice<Q>::c(s, 0), cy = Lattice<Q>::c(s, 1), cz = Lattice<Q>::c(s, 2);
-      int64_t idx;
       // Row-aligned accesses stream past the caches; x-shifted ones (c_x != 0 in
       // the gathering kinds) share their boundary cache line with the
       // neighbouring wavefront and must stay cacheable.
       bool aligned = true;
       int cross = 0;  // pressure-drop boundary crossed by this gather
+      if constexpr (ROWS) {
+        int ss = s, xx = x, yy = y, zz = z;
+        if constexpr (KIND == K_AA_EVEN) {
+          ss = o;
+        } else if constexpr (KIND == K_AA_ODD || KIND == K_AB_PULL) {
+          aligned = cx == 0;
+          xx = wrap1(x - cx, a.nx);
+          yy = wrap1(y - cy, a.ny);
+          zz = wrap1(z - cz, a.nz);
+          cross = gather_crossing(cx, x, a.nx);
+        }
+        const auto* rp = row_of(f0, n, ss, a, yy, zz);
+        if (aligned) d[s] = ld_row<STORE, NT>(rp, (uint32_t)xx, a.inv_scale);
+        else d[s] = ld_row<STORE, false>(rp, (uint32_t)xx, a.inv_scale);
+        d[s] = with_jump<Q>(d[s], cross, s, a.djump);
+        continue;
+      }
+      int64_t idx;
       if constexpr (KIND == K_AA_EVEN) {
         idx = slot<Q, LAYOUT>(n, o, cell);
       } else if constexpr (KIND == K_AB_PUSH) {
@@ -281,6 +352,26 @@ __global__ void __launch_bounds__(kDenseMaxBlock)
     collide_cell<Q, STORE, FORCE, ONE>(d, om, F, a.gamma_field, cell);
   }
 
+  if constexpr (ROWS) {
+    auto put_row = [&](typename Storage<STORE>::T* rp, int xx, C v, bool aligned) {
+      if (aligned) st_row<STORE, NT>(rp, (uint32_t)xx, v, a.scale);
+      else st_row<STORE, false>(rp, (uint32_t)xx, v, a.scale);
+    };
+#pragma unroll
+    for (int s = 0; s < Q; s++) {
+      const int o = Lattice<Q>::opp(s);
+      const int cx = Lattice<Q>::c(s, 0), cy = Lattice<Q>::c(s, 1), cz = Lattice<Q>::c(s, 2);
+      if constexpr (KIND == K_AA_EVEN || KIND == K_AB_PULL) {
+        put_row(row_of(f1, n, s, a, y, z), x, d[s], true);
+      } else {
+        // streaming target: AA odd writes slot opp(s) at x + c_s, AB push slot s
+        put_row(row_of(f1, n, KIND == K_AA_ODD ? o : s, a, wrap1(y + cy, a.ny), wrap1(z + cz, a.nz)),
+                wrap1(x + cx, a.nx), with_jump<Q>(d[s], scatter_crossing(cx, x, a.nx), s, a.djump),
+                cx == 0);
+      }
+    }
+    return;
+  }
   auto put = [&](int64_t idx, C v, bool aligned) {
     if (aligned && LAYOUT == LBM_LAYOUT_SOA) st_s<STORE, NT>(f1, idx, v, a.scale);
     else st<STORE>(f1, idx, v, a.scale);
@@ -363,9 +454,8 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, kDenseMaxBlock),
   C* lds = reinterpret_cast<C*>(lbm_smem);  // [NSH][nx]
   const int nx = a.nx;
   const int x = threadIdx.x;
-  const int y = blockIdx.y, z = blockIdx.z;
+  const int y = blockIdx.y, z = blockIdx.z + a.z0;
   const int64_t cell = x + (int64_t)nx * (y + (int64_t)a.ny * z);
-  const Offsets off(x, y, z, a.nx, a.ny, a.nz);
   const int64_t n = a.ncell;
   constexpr bool GATHER = KIND == K_AA_ODD || KIND == K_AB_PULL;
   constexpr bool SCATTER = KIND == K_AA_ODD || KIND == K_AB_PUSH;
@@ -378,12 +468,12 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, kDenseMaxBlock),
       const int cx = Lattice<Q>::c(s, 0), cy = Lattice<Q>::c(s, 1), cz = Lattice<Q>::c(s, 2);
       if constexpr (GATHER) {
         // element x of slot s in row (y - cy, z - cz)
-        const C v = ld_s<STORE, NT>(f0, slot<Q, LBM_LAYOUT_SOA>(n, s, cell + off.at(0, -cy, -cz)),
-                                 a.inv_scale);
+        const C v = ld_row<STORE, NT>(
+            row_of(f0, n, s, a, wrap1(y - cy, a.ny), wrap1(z - cz, a.nz)), (uint32_t)x, a.inv_scale);
         if (cx == 0) d[s] = v;
         else lds[(k++) * nx + x] = v;
       } else {
-        d[s] = ld_s<STORE, NT>(f0, slot<Q, LBM_LAYOUT_SOA>(n, s, cell), a.inv_scale);
+        d[s] = ld_row<STORE, NT>(row_of(f0, n, s, a, y, z), (uint32_t)x, a.inv_scale);
       }
     }
   }
@@ -409,7 +499,8 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, kDenseMaxBlock),
 #pragma unroll
     for (int s = 0; s < Q; s++) {
       const int o = Lattice<Q>::opp(s);
-      if (!solid && ((lk >> o) & 1u)) d[s] = ld_s<STORE, NT>(f0, slot<Q, LBM_LAYOUT_SOA>(n, o, cell), a.inv_scale);
+      if (!solid && ((lk >> o) & 1u))
+        d[s] = ld_row<STORE, NT>(row_of(f0, n, o, a, y, z), (uint32_t)x, a.inv_scale);
     }
   }
 
@@ -428,7 +519,8 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, kDenseMaxBlock),
       const int cx = Lattice<Q>::c(s, 0), cy = Lattice<Q>::c(s, 1), cz = Lattice<Q>::c(s, 2);
       const int tgt = KIND == K_AA_ODD ? o : s;
       if (cx == 0) {
-        st_s<STORE, NT>(f1, slot<Q, LBM_LAYOUT_SOA>(n, tgt, cell + off.at(0, cy, cz)), d[s], a.scale);
+        st_row<STORE, NT>(row_of(f1, n, tgt, a, wrap1(y + cy, a.ny), wrap1(z + cz, a.nz)),
+                          (uint32_t)x, d[s], a.scale);
       } else {
         int xt = x + cx;
         xt = xt < 0 ? xt + nx : (xt >= nx ? xt - nx : xt);
@@ -443,15 +535,181 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, kDenseMaxBlock),
       const int cx = Lattice<Q>::c(s, 0), cy = Lattice<Q>::c(s, 1), cz = Lattice<Q>::c(s, 2);
       const int tgt = KIND == K_AA_ODD ? o : s;
       if (cx != 0)
-        st_s<STORE, NT>(f1, slot<Q, LBM_LAYOUT_SOA>(n, tgt, cell + off.at(0, cy, cz)),
-                     lds[(k++) * nx + x], a.scale);
+        st_row<STORE, NT>(row_of(f1, n, tgt, a, wrap1(y + cy, a.ny), wrap1(z + cz, a.nz)),
+                          (uint32_t)x, lds[(k++) * nx + x], a.scale);
     }
   } else {
     if (solid && !a.fill_solid) return;
 #pragma unroll
     for (int s = 0; s < Q; s++)
-      st_s<STORE, NT>(f1, slot<Q, LBM_LAYOUT_SOA>(n, s, cell), solid ? C(0) : d[s], a.scale);
+      st_row<STORE, NT>(row_of(f1, n, s, a, y, z), (uint32_t)x, solid ? C(0) : d[s], a.scale);
   }
+}
+
+// Segmented LDS-staged AA odd step for fully periodic SOA boxes: a workgroup
+// is a segment of W = blockDim.x cells of one x-row (W a multiple of 64 that
+// divides nx), so the odd step can run the small, capped workgroups that made
+// the even step fast, instead of whole rows.  As in k_dense_staged, every
+// access of a segment is its aligned element x of some row, and the x-shift of
+// the c_x != 0 directions is applied in LDS; what a segment needs beyond its
+// own W elements is one element per shifted direction:
+//   gather : node x0 (c_x = +1) reads element x0 - 1 and node x0 + W - 1
+//            (c_x = -1) element x0 + W of the source row; lane k < NSH loads
+//            direction k's element into the LDS halo;
+//   scatter: node x0 + W - 1 (c_x = +1) writes element x0 + W and node x0
+//            (c_x = -1) element x0 - 1 of the target row; lane k stores it
+//            from the LDS halo, and the owner segment skips that element.
+// Each such element is read and written only by the one node of this segment
+// whose AA read set and write set it belongs to (a node reads and writes the
+// same Q slots), so neighbouring segments never touch it: the update is race
+// free in place, and its values are the whole-row kernel's bit for bit.  The
+// halo element shares a cache line with the neighbouring segment's aligned
+// accesses; the segments of one row are placed on one XCD (seg_coords) so
+// that the line meets in that XCD's L2, and the halo accesses stay cacheable.
+//
+// Linear block b -> (row, segment): the hardware hands blocks to the 8 XCDs in
+// turn (b mod 8), so with t = b / 8 block b takes segment t mod nseg of row
+// (t / nseg) * 8 + b mod 8: the nseg segments of a row are consecutive blocks
+// of one XCD.  Needs the number of rows to be a multiple of 8 (else plain
+// order: row b / nseg, segment b mod nseg).
+__device__ __forceinline__ void seg_coords(int nseg, int nrows, int& row, int& seg) {
+  // rows are numbered y + ny * (z - z0) over the planes of the launch
+  const int b = blockIdx.x;
+  if (nrows % 8 == 0) {
+    const int t = b / 8;
+    seg = t % nseg;
+    row = (t / nseg) * 8 + b % 8;
+  } else {
+    seg = b % nseg;
+    row = b / nseg;
+  }
+}
+
+template <int Q, int STORE, bool FORCE, bool NT = true,
+          bool ONE = one_collision_body<Q, STORE, K_AA_ODD, FORCE>()>
+__global__ void __launch_bounds__(256) k_dense_seg_odd(typename Storage<STORE>::T* f, DenseArgs a) {
+  using T = typename Storage<STORE>::T;
+  using C = typename Storage<STORE>::C;
+  constexpr int NSH = n_xshift<Q>();
+  extern __shared__ unsigned char lbm_smem[];
+  const int W = blockDim.x;
+  C* lds = reinterpret_cast<C*>(lbm_smem);  // [NSH][W + 2]: halo, W elements, halo
+  const int nx = a.nx;
+  const int nseg = nx / W;
+  int row, seg;
+  seg_coords(nseg, (int)gridDim.x / nseg, row, seg);  // one block per segment of every row
+  const int y = row % a.ny, z = row / a.ny + a.z0;
+  const int tid = threadIdx.x;
+  const int x0 = seg * W, x = x0 + tid;
+  const int xl = x0 == 0 ? nx - 1 : x0 - 1, xr = x0 + W == nx ? 0 : x0 + W;  // halo positions
+  const int64_t cell = x + (int64_t)nx * (y + (int64_t)a.ny * z);
+  const int64_t n = a.ncell;
+  const int L = W + 2;
+
+  // Gather.  All loads are issued before any LDS store: lane k < NSH also
+  // loads direction k's halo element, through an address selected per lane (a
+  // load inside a lane-divergent branch would make the wave wait for every
+  // load issued before it).
+  const T* hp = f + cell;  // the halo element of lane k < NSH (lanes >= NSH: own element)
+  int hpos = 1 + tid;      // its LDS position
+  {
+    int k = 0;
+#pragma unroll
+    for (int s = 0; s < Q; s++) {
+      const int cx = Lattice<Q>::c(s, 0), cy = Lattice<Q>::c(s, 1), cz = Lattice<Q>::c(s, 2);
+      if (cx != 0) {
+        if (tid == k) {
+          hp = row_of(f, n, s, a, wrap1(y - cy, a.ny), wrap1(z - cz, a.nz)) + (cx > 0 ? xl : xr);
+          hpos = k * L + (cx > 0 ? 0 : W + 1);
+        }
+        k++;
+      }
+    }
+  }
+  const C hv = ld<STORE>(hp, 0, a.inv_scale);
+  // keep the halo load ahead of the aligned ones: scheduled after them, its
+  // latency would be waited out on its own before the first barrier
+  __builtin_amdgcn_sched_barrier(0);
+  C d[Q];
+  C v[NSH];
+  {
+    int k = 0;
+#pragma unroll
+    for (int s = 0; s < Q; s++) {
+      const int cx = Lattice<Q>::c(s, 0), cy = Lattice<Q>::c(s, 1), cz = Lattice<Q>::c(s, 2);
+      // element x of slot s in row (y - cy, z - cz)
+      const T* rp = row_of(f, n, s, a, wrap1(y - cy, a.ny), wrap1(z - cz, a.nz));
+      const C val = ld_row<STORE, NT>(rp, (uint32_t)x, a.inv_scale);
+      if (cx == 0) d[s] = val;
+      else v[k++] = val;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NSH; k++) lds[k * L + 1 + tid] = v[k];
+  if (tid < NSH) lds[hpos] = hv;
+  __syncthreads();
+  {
+    int k = 0;
+#pragma unroll
+    for (int s = 0; s < Q; s++) {
+      const int cx = Lattice<Q>::c(s, 0);
+      if (cx != 0) {
+        d[s] = with_jump<Q>(lds[k * L + 1 + tid - cx], gather_crossing(cx, x, nx), s, a.djump);
+        k++;
+      }
+    }
+  }
+  {
+    const double om = a.omega_field ? a.omega_field[cell] : a.omega;
+    const double F[3] = {a.F0, a.F1, a.F2};
+    collide_cell<Q, STORE, FORCE, ONE>(d, om, F, a.gamma_field, cell);
+  }
+  __syncthreads();  // every LDS read of the gather is done
+  {
+    int k = 0;
+#pragma unroll
+    for (int s = 0; s < Q; s++) {
+      const int o = Lattice<Q>::opp(s);
+      const int cx = Lattice<Q>::c(s, 0), cy = Lattice<Q>::c(s, 1), cz = Lattice<Q>::c(s, 2);
+      if (cx == 0) {
+        st_row<STORE, NT>(row_of(f, n, o, a, wrap1(y + cy, a.ny), wrap1(z + cz, a.nz)), (uint32_t)x,
+                          d[s], a.scale);
+      } else {
+        lds[k * L + 1 + tid + cx] = with_jump<Q>(d[s], scatter_crossing(cx, x, nx), s, a.djump);
+        k++;
+      }
+    }
+  }
+  __syncthreads();
+  // Scatter: aligned elements, then lane k < NSH stores direction k's halo
+  // element (through a per-lane address, as in the gather).
+  {
+    T* sp = f + cell;
+    int spos = 0;
+    int k = 0;
+#pragma unroll
+    for (int s = 0; s < Q; s++) {
+      const int o = Lattice<Q>::opp(s);
+      const int cx = Lattice<Q>::c(s, 0), cy = Lattice<Q>::c(s, 1), cz = Lattice<Q>::c(s, 2);
+      if (cx != 0) {
+        T* rp = row_of(f, n, o, a, wrap1(y + cy, a.ny), wrap1(z + cz, a.nz));
+        // the element the neighbouring segment's node writes through its halo
+        const bool foreign = (cx > 0 && tid == 0) || (cx < 0 && tid == W - 1);
+        if (!foreign) st_row<STORE, NT>(rp, (uint32_t)x, lds[k * L + 1 + tid], a.scale);
+        if (tid == k) {
+          sp = rp + (cx > 0 ? xr : xl);
+          spos = k * L + (cx > 0 ? W + 1 : 0);
+        }
+        k++;
+      }
+    }
+    if (tid < NSH) st<STORE>(sp, 0, lds[spos], a.scale);
+  }
+}
+
+template <int Q, int STORE>
+constexpr size_t seg_lds_bytes(int w) {
+  return (size_t)n_xshift<Q>() * (w + 2) * sizeof(typename Storage<STORE>::C);
 }
 
 template <int Q, int STORE>
@@ -471,6 +729,37 @@ void launch_dense(dim3 grid, dim3 block, hipStream_t stream, void* f0, void* f1,
   using T = typename Storage<STORE>::T;
   hipLaunchKernelGGL((k_dense<Q, STORE, LAYOUT, KIND, MASK, FORCE, NT>), grid, block, lds_min, stream,
                      (const T*)f0, (T*)f1, a);
+}
+
+template <int Q, int STORE, bool FORCE, bool NT>
+void launch_seg_odd(dim3 grid, dim3 block, hipStream_t stream, void* f0, void* f1,
+                    const DenseArgs& a, size_t lds_min) {
+  using T = typename Storage<STORE>::T;
+  (void)f0;  // in place: f0 == f1
+  size_t lds = seg_lds_bytes<Q, STORE>((int)block.x);
+  if (lds < lds_min) lds = lds_min;
+  hipLaunchKernelGGL((k_dense_seg_odd<Q, STORE, FORCE, NT>), grid, block, lds, stream, (T*)f1, a);
+}
+
+template <int Q, bool NT>
+DenseLaunchFn pick_seg_odd_qn(int storage, bool force) {
+  switch (storage) {
+    case LBM_STORE_F64:
+      return force ? launch_seg_odd<Q, LBM_STORE_F64, true, NT>
+                   : launch_seg_odd<Q, LBM_STORE_F64, false, NT>;
+    case LBM_STORE_F32S:
+      return force ? launch_seg_odd<Q, LBM_STORE_F32S, true, NT>
+                   : launch_seg_odd<Q, LBM_STORE_F32S, false, NT>;
+    case LBM_STORE_F16S:
+      return force ? launch_seg_odd<Q, LBM_STORE_F16S, true, NT>
+                   : launch_seg_odd<Q, LBM_STORE_F16S, false, NT>;
+  }
+  return nullptr;
+}
+
+template <int Q>
+DenseLaunchFn pick_seg_odd_q(int storage, bool force, bool nt) {
+  return nt ? pick_seg_odd_qn<Q, true>(storage, force) : pick_seg_odd_qn<Q, false>(storage, force);
 }
 
 template <int Q, int STORE, int KIND, bool FORCE, bool MASK, bool NT>
@@ -580,5 +869,8 @@ DenseLaunchFn pick_staged19(int storage, int kind, bool force, int nx, bool mask
                             bool nt);
 DenseLaunchFn pick_staged27(int storage, int kind, bool force, int nx, bool mask, size_t lds_max,
                             bool nt);
+// segmented LDS-staged AA odd step (k_dense_seg_odd): grid = one block per row segment
+DenseLaunchFn pick_seg_odd19(int storage, bool force, bool nt);
+DenseLaunchFn pick_seg_odd27(int storage, bool force, bool nt);
 
 }  // namespace lbm

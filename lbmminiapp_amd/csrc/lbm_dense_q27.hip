@@ -10,4 +10,7 @@ DenseLaunchFn pick_staged27(int storage, int kind, bool force, int nx, bool mask
                             size_t lds_max, bool nt) {
   return pick_staged_q<27>(storage, kind, force, nx, mask, lds_max, nt);
 }
+DenseLaunchFn pick_seg_odd27(int storage, bool force, bool nt) {
+  return pick_seg_odd_q<27>(storage, force, nt);
+}
 }  // namespace lbm

@@ -245,3 +245,31 @@ def test_adjoint_lambda0_vs_oracle(steps):
                                   gamma_node=gamma[g.cell_of_fluid], with_lambda0=True)
     assert np.abs(lam[:, g.cell_of_fluid] - lam_o).max() <= 1e-12 * np.abs(lam_o).max()
     assert not lam[:, solid != 0].any()
+
+
+@pytest.mark.parametrize("param", ["gamma", "tau"])
+def test_directional_difference_matches_adjoint(param):
+    """lbm_adjoint_fd_direction: the 6th-order central difference of J along a
+    seeded whole-field direction equals sum_x grad(x) v(x) from the GPU adjoint
+    (and from the oracle adjoint) to 1e-7 relative, for both fields and the
+    kinetic-energy objective on the small channel."""
+    q = 19
+    g, solid, gamma, tau, fc, full = small_channel(q, seed=7)
+    steps, F = 12, (2e-5, 0.0, 0.0)
+    v = O.uniform01(91, np.arange(g.ncell, dtype=np.uint64)) - 0.5
+    with Adjoint(g.nx, g.ny, g.nz, q=q, storage="f64", omega=1.0, force=F, solid=solid, tau=tau,
+                 gamma=gamma, checkpoint_every=5) as adj:
+        adj.set_initial_state(full)
+        adj.forward(steps)
+        gg, gt, _ = adj.backward()
+        d = adj.finite_difference_direction(param, v, 1e-3, steps, order=6)
+        J_after = adj.forward(steps)  # the field is restored after the differences
+    _, ggo, gto, _ = O.adjoint(q, g, fc, steps, 1.0, F, tau_node=tau[g.cell_of_fluid],
+                               gamma_node=gamma[g.cell_of_fluid])
+    fl = g.cell_of_fluid
+    grad, grad_o = (gg, ggo) if param == "gamma" else (gt, gto)
+    pred = float(np.dot(grad[fl], v[fl]))
+    assert abs(pred - float(np.dot(grad_o, v[fl]))) <= 1e-10 * abs(pred)
+    assert abs(pred - d) <= 1e-7 * abs(d), (pred, d)
+    Jo = O.adjoint(q, g, fc, steps, 1.0, F, tau_node=tau[fl], gamma_node=gamma[fl])[0]
+    assert abs(J_after - Jo) <= 1e-13 * abs(Jo)

@@ -394,12 +394,14 @@ def test_dense_init_tgv_and_moments(q):
     assert np.abs(vel - (c.T @ fc / r2).T).max() < 1e-15
 
 
-@pytest.mark.parametrize("storage,tol_u", [("f32s", 2e-5), ("f16s", 5e-3)])
+@pytest.mark.parametrize("storage,tol_u,tol_rho", [("f32s", 2e-5, 1e-7), ("f16s", 1e-3, 1e-5)])
 @pytest.mark.parametrize("q", [19, 27])
-def test_dense_shifted_storage_error(storage, tol_u, q):
+def test_dense_shifted_storage_error(storage, tol_u, tol_rho, q):
     """FP32 / FP16 deviation storage (FP32 collision, FP64 moment sums) against
-    the FP64 run: velocity rel-L2 error after 200 steps of a 64^3 Taylor-Green
-    vortex (tau = 0.6), mass drift, and exact rest-state preservation."""
+    the FP64 run after 200 steps of a 64^3 Taylor-Green vortex (tau = 0.6):
+    velocity rel-L2 within 2e-5 (FP32) and 1e-3 (FP16, the bar the 2000-step
+    512^3 tests hold too), density rel-L2 within 1e-7 and 1e-5 (measured on the
+    MI355X: u 3.5e-6 / 1.6e-4, rho 7e-9 / 2.8e-6)."""
     n, steps = 64, 200
     out = {}
     for st in ("f64", storage):
@@ -416,7 +418,7 @@ def test_dense_shifted_storage_error(storage, tol_u, q):
     print(f"[dense] {storage} q={q}: u rel-L2 {err_u:.3e}, rho rel-L2 {err_rho:.3e}, "
           f"mass drift {(m1 - m0) / m0:.3e}")
     assert err_u < tol_u
-    assert err_rho < tol_u
+    assert err_rho < tol_rho
 
 
 @pytest.mark.parametrize("pattern", ["aa", "push", "pull"])
@@ -504,3 +506,85 @@ def test_dense_aos_shifted_storage_vs_oracle(q, pattern, storage, masked, force)
     assert np.abs(got["aos"] - got["soa"]).max() < tol
     if masked:
         assert not got["aos"][:, np.nonzero(solid)[0]].any()
+
+
+@pytest.mark.parametrize("planes", [1, 2, 3, 5, 16])
+@pytest.mark.parametrize("q,storage,masked", [(19, "f64", False), (19, "f64", True),
+                                              (27, "f64", False), (19, "f32s", False)])
+def test_dense_aa_slab_schedule(planes, q, storage, masked, monkeypatch):
+    """AA step pairs run as z-slabs of `planes` planes (even(S-1), even(0),
+    then even(k+1), odd(k) ..., odd(S-1)); nz = 11 is not a multiple of most
+    slab sizes and 16 planes make a single slab.  Rows of 128 (the staged odd
+    kernel on periodic boxes), a masked channel (unstaged kernels), 8 and 9
+    steps (a trailing odd step runs whole).  FP64 equals the oracle bit for
+    bit, FP32 deviations within 1e-7."""
+    monkeypatch.setenv("LBM_DENSE_SLAB_PLANES", str(planes))
+    nx, ny, nz = 128, 6, 11
+    solid, periodic = None, (1, 1, 1)
+    if masked:
+        solid = (O.uniform01(81, np.arange(nx * ny * nz, dtype=np.uint64)) < 0.2).astype(np.uint8)
+        periodic = (1, 0, 1)
+    geom = O.DenseGeometry(nx, ny, nz, solid, periodic=periodic)
+    fc = seeded_fc(geom, q, seed=33)
+    omega, F = 1 / 0.6, (1e-6, 0.0, 0.0)
+    for steps in (8, 9):
+        print(f"[dense] slabs of {planes} q={q} {storage} masked={masked} steps={steps}", flush=True)
+        with DenseLattice(nx, ny, nz, q=q, pattern="aa", storage=storage, periodic=periodic,
+                          omega=omega, force=F, solid=solid) as lat:
+            lat.set_state(dense_canonical(geom, q, fc))
+            lat.step(3)
+            lat.step(steps - 3)
+            got = lat.get_state()[:, geom.cell_of_fluid]
+        want = oracle_run(geom, q, fc, "aa", steps, omega, F)
+        if storage == "f64":
+            assert np.array_equal(got, want), steps
+        else:
+            assert np.abs(got - want).max() < 1e-7, steps
+
+
+@pytest.mark.parametrize("w", [64, 128, 256])
+@pytest.mark.parametrize("q,storage", [(19, "f64"), (27, "f64"), (19, "f32s"), (19, "f16s"),
+                                       (27, "f16s")])
+@pytest.mark.parametrize("rows", [(6, 5), (8, 4)])
+def test_dense_segmented_odd_step(w, q, storage, rows, monkeypatch):
+    """The AA odd step as row segments of w cells (k_dense_seg_odd): rows of
+    256 cells in 4, 2 or 1 segments, 30 rows (plain block order) and 32 rows
+    (the segments of a row on one XCD), with body force, after 7 steps and
+    with a pressure-drop jump across the periodic x boundary.  FP64 equals
+    the oracle bit for bit; FP32 / FP16 deviations within 1e-7 / 2e-5."""
+    monkeypatch.setenv("LBM_DENSE_ODD_SEG", str(w))
+    nx, (ny, nz) = 256, rows
+    geom = O.DenseGeometry(nx, ny, nz)
+    fc = seeded_fc(geom, q, seed=44)
+    omega, F = 1 / 0.6, (1e-5, -2e-6, 3e-6)
+    print(f"[dense] segmented odd w={w} q={q} {storage} rows={rows}", flush=True)
+    tol = {"f64": 0.0, "f32s": 1e-7, "f16s": 2e-5}[storage]
+    for jump in (0.0, 3e-3):
+        with DenseLattice(nx, ny, nz, q=q, pattern="aa", storage=storage, omega=omega, force=F,
+                          density_jump_x=jump) as lat:
+            lat.set_state(fc)
+            lat.step(7)
+            got = lat.get_state()
+        if jump == 0.0:
+            want = oracle_run(geom, q, fc, "aa", 7, omega, F)
+        else:
+            want = oracle_run_jump(geom, q, fc, "aa", 7, omega, F, jump)
+        if storage == "f64":
+            assert np.array_equal(got, want), jump
+        else:
+            assert np.abs(got - want).max() < tol, jump
+
+
+def test_dense_segmented_odd_with_slabs(monkeypatch):
+    """Segmented odd step inside the z-slab schedule (z offsets per launch)."""
+    monkeypatch.setenv("LBM_DENSE_ODD_SEG", "128")
+    monkeypatch.setenv("LBM_DENSE_SLAB_PLANES", "3")
+    nx, ny, nz = 256, 8, 7
+    geom = O.DenseGeometry(nx, ny, nz)
+    fc = seeded_fc(geom, 19, seed=45)
+    with DenseLattice(nx, ny, nz, q=19, pattern="aa", storage="f64", omega=1 / 0.7,
+                      force=(1e-6, 0, 0)) as lat:
+        lat.set_state(fc)
+        lat.step(9)
+        got = lat.get_state()
+    assert np.array_equal(got, oracle_run(geom, 19, fc, "aa", 9, 1 / 0.7, (1e-6, 0, 0)))
