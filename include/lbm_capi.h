@@ -123,6 +123,19 @@ lbm_status lbm_sparse_init_rest(lbm_sparse* h, void* dstrb, int32_t aa);
  * LBM_ERR_NUMERIC if it is not finite. */
 lbm_status lbm_sparse_mass(lbm_sparse* h, const void* dstrb, int64_t start, int64_t count,
                            double* mass);
+/* lbm_sparse_mass without any allocation.  No counterpart in the reference;
+ * for callers that cannot allocate device memory: a time loop captured as a
+ * HIP graph on the stream set with lbm_sparse_set_stream, serving loops,
+ * harnesses that forbid hipMalloc.  Same kernels and the same bits as
+ * lbm_sparse_mass, but the partial sums go to a fixed per-GPU scratch array
+ * of 4096 doubles (32 KiB, held by the library until the process ends), and
+ * the result is written to `mass_device` (device, managed or pinned host
+ * memory) on the handle's stream without waiting and without the finiteness
+ * check.  count may be at most 16,777,216 nodes (4096 chunks of 4096);
+ * larger ranges return LBM_ERR_ARG.  Calls that overlap on one GPU (other
+ * streams, other threads) share the scratch and race: order them. */
+lbm_status lbm_sparse_mass_noalloc(lbm_sparse* h, const void* dstrb, int64_t start,
+                                   int64_t count, double* mass_device);
 /* Stream-order point: waits for every launch issued on the handle. */
 lbm_status lbm_sparse_synchronize(lbm_sparse* h);
 /* Enqueue later launches on `stream` (a hipStream_t; NULL, the default, is the

@@ -27,8 +27,8 @@ EXPORTED = [
     "lbm_last_error", "lbm_version", "lbm_device_count", "lbm_set_device", "lbm_device_alloc",
     "lbm_device_free", "lbm_memcpy_h2d", "lbm_memcpy_d2h", "lbm_device_synchronize",
     "lbm_sparse_create", "lbm_sparse_destroy", "lbm_sparse_aa_even", "lbm_sparse_aa_odd",
-    "lbm_sparse_ab", "lbm_sparse_init_rest", "lbm_sparse_mass", "lbm_sparse_synchronize",
-    "lbm_sparse_set_stream",
+    "lbm_sparse_ab", "lbm_sparse_init_rest", "lbm_sparse_mass", "lbm_sparse_mass_noalloc",
+    "lbm_sparse_synchronize", "lbm_sparse_set_stream",
     "lbm_dense_create", "lbm_dense_destroy", "lbm_dense_get_info", "lbm_dense_set_state",
     "lbm_dense_get_state", "lbm_dense_init_equilibrium", "lbm_dense_init_tgv", "lbm_dense_step",
     "lbm_dense_run_timed", "lbm_dense_moments", "lbm_dense_mass", "lbm_dense_synchronize",
@@ -105,7 +105,7 @@ def load(path: str = LIB_PATH):
         "lbm_sparse_create": [P, P], "lbm_sparse_destroy": [P],
         "lbm_sparse_aa_even": [P, P, I64, I64], "lbm_sparse_aa_odd": [P, P, I64, I64],
         "lbm_sparse_ab": [P, P, P, I64, I64, I32], "lbm_sparse_init_rest": [P, P, I32],
-        "lbm_sparse_mass": [P, P, I64, I64, P],
+        "lbm_sparse_mass": [P, P, I64, I64, P], "lbm_sparse_mass_noalloc": [P, P, I64, I64, P],
         "lbm_sparse_synchronize": [P], "lbm_sparse_set_stream": [P, P],
         "lbm_dense_create": [P, P], "lbm_dense_destroy": [P], "lbm_dense_get_info": [P, P],
         "lbm_dense_set_state": [P, P], "lbm_dense_get_state": [P, P],
@@ -297,6 +297,12 @@ class SparseKernels:
         count = self.n - start if count is None else count
         check(load().lbm_sparse_mass(self.h, buf.ptr, int(start), int(count), ctypes.byref(m)))
         return m.value
+
+    def mass_noalloc(self, buf: DeviceBuffer, out: DeviceBuffer, start=0, count=None):
+        """lbm_sparse_mass_noalloc: enqueues the FP64 sum into the device double
+        `out` (no allocation, no wait)."""
+        count = self.n - start if count is None else count
+        check(load().lbm_sparse_mass_noalloc(self.h, buf.ptr, int(start), int(count), out.ptr))
 
     def synchronize(self):
         check(load().lbm_sparse_synchronize(self.h))

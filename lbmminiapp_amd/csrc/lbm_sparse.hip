@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <mutex>
 
 #include "lbm_collide.hpp"
 #include "lbm_internal.hpp"
@@ -187,6 +188,17 @@ __global__ void __launch_bounds__(kMassBlock)
   if (threadIdx.x == 0) *out = tot;
 }
 
+// Partial sums of lbm_sparse_mass_noalloc: a fixed device array instead of an
+// allocation, kMassScratchPartials doubles (32 KiB) on every GPU the library
+// runs on, held until the process ends.  It bounds that entry point's range to
+// kMassScratchPartials * kMassChunk = 16,777,216 nodes.  In host code `scratch`
+// only names the array; its device address comes from hipGetSymbolAddress
+// (scratch_on_current_device), once per device.  The two-pass reduction
+// writes every partial it reads, so nothing needs clearing between calls;
+// calls that overlap on one GPU share the array and race.
+constexpr int64_t kMassScratchPartials = 4096;
+static __device__ __align__(256) double scratch[kMassScratchPartials];
+
 }  // namespace lbm
 
 struct lbm_sparse {
@@ -338,6 +350,39 @@ lbm_status launch_mass(const lbm_sparse* h, const void* dstrb, int64_t start, in
 
 int64_t mass_blocks(int64_t count) { return (count + kMassChunk - 1) / kMassChunk; }
 
+// Device address of `scratch` on the current device, looked up once per device
+// (at lbm_sparse_create, or at the first allocation-free call on a device).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+lbm_status scratch_on_current_device(double** out) {
+  int dev = 0;
+  LBM_HIP_CHECK(hipGetDevice(&dev));
+  LBM_REQUIRE(dev >= 0 && dev < kMaxDevices, LBM_ERR_UNSUPPORTED, "device %d beyond %d", dev,
+              kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    LBM_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    LBM_REQUIRE(p, LBM_ERR_HIP, "no device address for the mass scratch on device %d", dev);
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return LBM_OK;
+}
+
+// true when the device can read and write *p: device, managed or pinned host memory
+bool device_accessible(const void* p) {
+  hipPointerAttribute_t attr{};
+  if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged ||
+         attr.type == hipMemoryTypeHost || attr.type == hipMemoryTypeUnified;
+}
+
 }  // namespace
 
 extern "C" {
@@ -383,6 +428,12 @@ lbm_status lbm_sparse_create(const lbm_sparse_config* cfg, lbm_sparse** out) {
     delete h;
     set_error("lbm_sparse_create: adjacency copy failed: %s", hipGetErrorString(e));
     return LBM_ERR_HIP;
+  }
+  double* scr = nullptr;  // set-up of lbm_sparse_mass_noalloc's scratch on this device
+  if (lbm_status st = scratch_on_current_device(&scr)) {
+    (void)hipFree(h->adj);
+    delete h;
+    return st;
   }
   *out = h;
   return LBM_OK;
@@ -498,6 +549,23 @@ lbm_status lbm_sparse_mass(lbm_sparse* h, const void* dstrb, int64_t start, int6
   LBM_REQUIRE(std::isfinite(*mass), LBM_ERR_NUMERIC, "lbm_sparse_mass: mass is not finite (%g)",
               *mass);
   return LBM_OK;
+}
+
+lbm_status lbm_sparse_mass_noalloc(lbm_sparse* h, const void* dstrb, int64_t start,
+                                   int64_t count, double* mass_device) {
+  LBM_REQUIRE(h && dstrb && mass_device, LBM_ERR_ARG, "lbm_sparse_mass_noalloc: null pointer");
+  if (lbm_status st = check_range(h, start, count)) return st;
+  const int64_t nb = mass_blocks(count);
+  LBM_REQUIRE(nb <= kMassScratchPartials, LBM_ERR_ARG,
+              "lbm_sparse_mass_noalloc: %lld nodes exceed the fixed scratch (at most %lld nodes "
+              "per call); use lbm_sparse_mass or split the range",
+              (long long)count, (long long)(kMassScratchPartials * kMassChunk));
+  LBM_REQUIRE(device_accessible(dstrb) && device_accessible(mass_device), LBM_ERR_ARG,
+              "lbm_sparse_mass_noalloc: the populations and the result must be device-accessible "
+              "memory (device, managed or pinned host)");
+  double* scr = nullptr;
+  if (lbm_status st = scratch_on_current_device(&scr)) return st;
+  return launch_mass(h, dstrb, start, count, scr, nb, mass_device);
 }
 
 lbm_status lbm_sparse_synchronize(lbm_sparse* h) {
