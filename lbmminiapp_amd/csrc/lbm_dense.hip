@@ -21,17 +21,25 @@ namespace lbm {
 
 enum CanonMode { C_LOCAL = 0, C_OPP = 1, C_GATHER = 2 };
 
+// storage cell of cell = (x, y, z): on interleaved rows element x of the row
+// sits at position ir_pos(x)
+__device__ __forceinline__ int64_t store_cell(const DenseArgs& a, int64_t cell, int x) {
+  return a.xk ? cell - x + ir_pos(x, a.xk) : cell;
+}
+
 // storage index of f_s(x) for a given state convention
 template <int Q, int LAYOUT>
-__device__ __forceinline__ int64_t canon_index(int mode, int s, int64_t cell, uint32_t lk,
-                                               const Offsets& off, int64_t n) {
+__device__ __forceinline__ int64_t canon_index(int mode, int s, int64_t cell, int x, uint32_t lk,
+                                               const Offsets& off, const DenseArgs& a) {
+  const int64_t n = a.ncell;
   const int o = Lattice<Q>::opp(s);
-  if (mode == C_LOCAL) return slot<Q, LAYOUT>(n, s, cell);
-  if (mode == C_OPP) return slot<Q, LAYOUT>(n, o, cell);
-  if ((lk >> o) & 1u) return slot<Q, LAYOUT>(n, o, cell);
-  return slot<Q, LAYOUT>(n, s,
-                         cell + off.at(-Lattice<Q>::c(s, 0), -Lattice<Q>::c(s, 1),
-                                       -Lattice<Q>::c(s, 2)));
+  if (mode == C_LOCAL) return slot<Q, LAYOUT>(n, s, store_cell(a, cell, x));
+  if (mode == C_OPP || ((lk >> o) & 1u)) return slot<Q, LAYOUT>(n, o, store_cell(a, cell, x));
+  const int cx = Lattice<Q>::c(s, 0);
+  return slot<Q, LAYOUT>(
+      n, s,
+      store_cell(a, cell + off.at(-cx, -Lattice<Q>::c(s, 1), -Lattice<Q>::c(s, 2)),
+                 wrap1(x - cx, a.nx)));
 }
 
 __device__ __forceinline__ bool cell_coords(const DenseArgs& a, int& x, int& y, int& z,
@@ -98,7 +106,7 @@ __global__ void __launch_bounds__(256)
     double v = with_jump<Q>(src[a.ncell * s + cell], -canon_jump_sign<Q>(mode, s, x, lk, a.nx), s,
                             a.djump);
     if constexpr (STORE != LBM_STORE_F64) v -= Lattice<Q>::w(s);
-    st<STORE>(f, canon_index<Q, LAYOUT>(mode, s, cell, lk, off, a.ncell),
+    st<STORE>(f, canon_index<Q, LAYOUT>(mode, s, cell, x, lk, off, a),
               (typename Storage<STORE>::C)v, a.scale);
   }
 }
@@ -122,7 +130,7 @@ __global__ void __launch_bounds__(256)
     double v = 0.0;
     if (!(lk & SOLID_BIT))
       v = with_jump<Q>(
-          to_f<STORE>(ld<STORE>(f, canon_index<Q, LAYOUT>(mode, s, cell, lk, off, a.ncell),
+          to_f<STORE>(ld<STORE>(f, canon_index<Q, LAYOUT>(mode, s, cell, x, lk, off, a),
                                 a.inv_scale),
                       Lattice<Q>::w(s)),
           canon_jump_sign<Q>(mode, s, x, lk, a.nx), s, a.djump);
@@ -146,7 +154,7 @@ __global__ void __launch_bounds__(256)
     const Offsets off(x, y, z, a.nx, a.ny, a.nz);
     for (int s = 0; s < Q; s++) {
       const double v = with_jump<Q>(
-          to_f<STORE>(ld<STORE>(f, canon_index<Q, LAYOUT>(mode, s, cell, lk, off, a.ncell),
+          to_f<STORE>(ld<STORE>(f, canon_index<Q, LAYOUT>(mode, s, cell, x, lk, off, a),
                                 a.inv_scale),
                       Lattice<Q>::w(s)),
           canon_jump_sign<Q>(mode, s, x, lk, a.nx), s, a.djump);
@@ -213,7 +221,7 @@ __global__ void __launch_bounds__(256)
     double feq = w * rho * (1.0 + cdotv + 0.5 * cdotv * cdotv - 0.5 * vdotv);
     if constexpr (STORE != LBM_STORE_F64) feq = w * ((rho - 1.0) + rho * (cdotv + 0.5 * cdotv * cdotv - 0.5 * vdotv));
     feq = with_jump<Q>(feq, -canon_jump_sign<Q>(mode, s, x, lk, a.nx), s, a.djump);
-    st<STORE>(f, canon_index<Q, LAYOUT>(mode, s, cell, lk, off, a.ncell),
+    st<STORE>(f, canon_index<Q, LAYOUT>(mode, s, cell, x, lk, off, a),
               (typename Storage<STORE>::C)feq, a.scale);
   }
 }
@@ -346,7 +354,7 @@ void set_launch_shapes(lbm_dense* h) {
   auto clamp_bx = [](int bx) { return bx < 64 ? 64 : (bx > kDenseMaxBlock ? kDenseMaxBlock : bx); };
   h->step_bx = clamp_bx(env_int("LBM_DENSE_BLOCK_X", default_step_bx(h->nx, h->storage)));
   h->step_lds = lds_for_blocks_per_cu(env_int("LBM_DENSE_BLOCKS_PER_CU", 0));
-  int even_bx = h->step_bx, even_k = 0;
+  int even_bx = h->step_bx, even_k = 0, odd_ir_bx = h->step_bx, odd_ir_k = 0;
   if (h->layout == LBM_LAYOUT_SOA) {
     static const int shape[2][3][2] = {
         // {width, workgroups per CU} for F64, F32S, F16S
@@ -356,6 +364,25 @@ void set_launch_shapes(lbm_dense* h) {
     const int* sh = shape[h->q == 19 ? 0 : 1][h->storage];
     even_bx = sh[0];
     even_k = sh[1];
+    // The AA odd step on interleaved rows makes only whole aligned block
+    // accesses, like the even step.  512^3, best of widths 64-512 and 0-8
+    // workgroups per CU; ms per step over 100-step runs, interleaved / plain
+    // rows (the plain-row odd step staged, segmented or unstaged as set
+    // below):
+    //   D3Q19 FP64   64 wide, <= 4 per CU: 7.11-7.14 / 7.20-7.24
+    //   D3Q19 FP32  256 wide, <= 4 per CU: 3.42-3.44 / 3.41-3.44
+    //   D3Q19 FP16  512 wide, no cap:      1.77-1.78 / 1.89-1.90
+    //   D3Q27 FP64  128 wide, <= 4 per CU: 10.62-10.77 / 10.75-10.96
+    //   D3Q27 FP16  512 wide, no cap:      2.68-2.69 / 2.87-2.89
+    // D3Q27 FP32 ran slower on interleaved rows (5.16-5.18 / 4.96-5.02) and
+    // keeps plain rows (dense_create).
+    static const int odd_shape[2][3][2] = {
+        {{64, 4}, {256, 4}, {512, 0}},   // D3Q19
+        {{128, 4}, {128, 4}, {512, 0}},  // D3Q27
+    };
+    const int* so = odd_shape[h->q == 19 ? 0 : 1][h->storage];
+    odd_ir_bx = so[0];
+    odd_ir_k = so[1];
   }
   // Non-temporal streaming pays when the populations exceed the 256 MiB
   // memory-side cache; a grid that fits there runs cached (LBM_DENSE_NT=0|1
@@ -372,6 +399,9 @@ void set_launch_shapes(lbm_dense* h) {
     h->lds_max = (size_t)lds;
   h->even_bx = clamp_bx(env_int("LBM_DENSE_EVEN_BLOCK_X", even_bx));
   h->even_lds = lds_for_blocks_per_cu(env_int("LBM_DENSE_EVEN_BLOCKS_PER_CU", even_k));
+  // LBM_DENSE_ODD_IR_BLOCK_X, LBM_DENSE_ODD_IR_BLOCKS_PER_CU: tuning
+  h->odd_ir_bx = clamp_bx(env_int("LBM_DENSE_ODD_IR_BLOCK_X", odd_ir_bx));
+  h->odd_ir_lds = lds_for_blocks_per_cu(env_int("LBM_DENSE_ODD_IR_BLOCKS_PER_CU", odd_ir_k));
   // Segmented odd step (k_dense_seg_odd): unmasked periodic SOA boxes whose
   // rows split into segments of a multiple of 64 cells.  512^3 D3Q19 odd step,
   // ms (whole-row staged / best segmented shape): FP32 deviations 3.55 / 3.34
@@ -428,7 +458,8 @@ DenseLaunchFn pick_step(const lbm_dense* h, int kind) {
 DenseLaunchFn pick_staged(const lbm_dense* h, int kind) {
   const int mode = h->staged_mode;
   const bool mask = h->links != nullptr;
-  if (mode == 0 || h->layout != LBM_LAYOUT_SOA || h->nx % 64 != 0 || h->nx > kDenseMaxBlock)
+  if (mode == 0 || h->layout != LBM_LAYOUT_SOA || h->nx % 64 != 0 || h->nx > kDenseMaxBlock ||
+      h->xk)
     return nullptr;
   if (mode < 2) {
     const bool f16 = h->storage == LBM_STORE_F16S;
@@ -448,6 +479,14 @@ lbm_status launch_kind(lbm_dense* h, int kind, void* f0, void* f1, int z0, int n
   }
   DenseArgs a = h->args;
   a.z0 = z0;
+  if (kind == K_AA_ODD && h->xk) {
+    DenseLaunchFn fn = pick_step(h, K_AA_ODD_IR);
+    LBM_REQUIRE(fn, LBM_ERR_UNSUPPORTED, "no interleaved-row odd kernel for this configuration");
+    const dim3 b(h->odd_ir_bx);
+    fn(dim3((h->nx + b.x - 1) / b.x, h->ny, nzl), b, h->stream, f0, f1, a, h->odd_ir_lds);
+    LBM_LAUNCH_CHECK();
+    return LBM_OK;
+  }
   if (kind == K_AA_ODD && h->odd_seg_w > 0) {
     const int w = h->odd_seg_w;
     const bool force = use_force_terms(h);
@@ -603,6 +642,29 @@ lbm_status dense_create(const lbm_dense_config* cfg, lbm_dense** out, bool push_
   const bool need_links =
       cfg->solid != nullptr || !h->periodic[0] || !h->periodic[1] || !h->periodic[2];
   if (need_links) h->odd_seg_w = 0;  // the segmented odd step covers unmasked periodic boxes
+  // Interleaved rows (lbm_dense_kernels.hpp) for AA on unmasked periodic SOA
+  // boxes whose rows are whole 64-element blocks: the odd step then makes only
+  // whole aligned block accesses, like the even step.  Per-cell fields are
+  // stored in the same order.  Default everywhere except D3Q27 FP32
+  // deviations, where plain rows measured faster (set_launch_shapes).
+  // LBM_DENSE_IROWS=0|1 overrides (A/B timing).
+  {
+    const char* e = std::getenv("LBM_DENSE_IROWS");
+    const bool on = e ? e[0] != '0' : !(h->q == 27 && h->storage == LBM_STORE_F32S);
+    if (on && h->pattern == LBM_PATTERN_AA && h->layout == LBM_LAYOUT_SOA && !need_links &&
+        h->nx % 64 == 0) {
+      h->xk = h->nx / 64;
+      h->odd_seg_w = 0;
+    }
+  }
+  // a per-cell field in the storage order of the populations
+  auto store_order = [h](std::vector<double>& v) {
+    if (!h->xk) return;
+    std::vector<double> t(v.size());
+    for (int64_t row = 0; row < h->ncell; row += h->nx)
+      for (int x = 0; x < h->nx; x++) t[row + ir_pos(x, h->xk)] = v[row + x];
+    v.swap(t);
+  };
   // AB push on a masked SOA grid runs the pull (gather) kernel one streaming
   // step out of phase: with P_0 = S^-1 f_0, (C S)^T P_0 = S^-1 (S C)^T f_0, so
   // the push state f_T is the gather of the pull storage, which is how every
@@ -677,6 +739,7 @@ lbm_status dense_create(const lbm_dense_config* cfg, lbm_dense** out, bool push_
       }
       om[c] = is_solid ? 1.0 : 1.0 / tau;
     }
+    store_order(om);
     if ((st = dmalloc(h, (void**)&h->omega_field, sizeof(double) * h->ncell, "omega field")))
       return fail(st);
     if (hipMemcpy(h->omega_field, om.data(), sizeof(double) * h->ncell, hipMemcpyHostToDevice) !=
@@ -688,8 +751,10 @@ lbm_status dense_create(const lbm_dense_config* cfg, lbm_dense** out, bool push_
   if (cfg->gamma) {
     if ((st = dmalloc(h, (void**)&h->gamma_field, sizeof(double) * h->ncell, "gamma field")))
       return fail(st);
-    if (hipMemcpy(h->gamma_field, cfg->gamma, sizeof(double) * h->ncell,
-                  hipMemcpyHostToDevice) != hipSuccess) {
+    std::vector<double> gm(cfg->gamma, cfg->gamma + h->ncell);
+    store_order(gm);
+    if (hipMemcpy(h->gamma_field, gm.data(), sizeof(double) * h->ncell, hipMemcpyHostToDevice) !=
+        hipSuccess) {
       set_error("lbm_dense_create: gamma field copy failed");
       return fail(LBM_ERR_HIP);
     }
@@ -719,18 +784,20 @@ lbm_status dense_create(const lbm_dense_config* cfg, lbm_dense** out, bool push_
   a.scale = (float)scale;
   a.inv_scale = (float)(1.0 / scale);
   a.djump = cfg->density_jump_x;
+  a.xk = h->xk;
   // With F = 0 the Guo terms are exact zeros, so the kernels that evaluate
   // them give the same bits as the specialised ones.  For D3Q19 FP64 AA on
-  // SOA boxes without solids and with rows of more than 256 cells (whole-row
-  // workgroups of more than 4 wavefronts) they also measured faster: two
-  // handles alternating in one process, MLUPS without / with the terms,
-  // 512^3: 17,436 / 18,304; 256^3: 18,437 / 18,414; 128^3: 19,083 / 18,965;
-  // 64^3: 18,373 / 16,990.  In every other case measured (FP32/FP16
-  // deviations, D3Q27, AB) they were equal or slower.  LBM_DENSE_FORCE_TERMS=0|1
+  // SOA boxes without solids and with plain rows of more than 256 cells
+  // (whole-row staged workgroups of more than 4 wavefronts) they also
+  // measured faster: two handles alternating in one process, MLUPS without /
+  // with the terms, 512^3: 17,436 / 18,304; 256^3: 18,437 / 18,414; 128^3:
+  // 19,083 / 18,965; 64^3: 18,373 / 16,990.  In every other case measured
+  // (FP32/FP16 deviations, D3Q27, AB, the interleaved-row odd step: 7.33 /
+  // 7.37 ms at 512^3) they were equal or slower.  LBM_DENSE_FORCE_TERMS=0|1
   // overrides.
   {
     const bool dflt = h->q == 19 && h->storage == LBM_STORE_F64 && h->pattern == LBM_PATTERN_AA &&
-                      h->layout == LBM_LAYOUT_SOA && h->links == nullptr && h->nx > 256;
+                      h->layout == LBM_LAYOUT_SOA && h->links == nullptr && h->nx > 256 && !h->xk;
     const char* e = std::getenv("LBM_DENSE_FORCE_TERMS");
     h->force_terms = e ? e[0] == '1' : dflt;
   }

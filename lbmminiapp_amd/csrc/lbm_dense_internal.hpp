@@ -18,6 +18,9 @@ struct lbm_dense {
   int slab_planes = 0;       // AA steps in pairs as z-slabs of this many planes (0: whole box)
   int odd_seg_w = 0;         // AA odd step as row segments of this width (k_dense_seg_odd; 0: off)
   size_t odd_seg_lds = 0;    // its dynamic LDS floor (caps workgroups per CU)
+  int xk = 0;                // interleaved rows: nx / 64 (0: plain rows)
+  int odd_ir_bx = 256;       // launch shape of the odd step on interleaved rows
+  size_t odd_ir_lds = 0;
   int64_t ncell = 0, nfluid = 0, t = 0;
   size_t esize = 8;
   void* f[2] = {nullptr, nullptr};

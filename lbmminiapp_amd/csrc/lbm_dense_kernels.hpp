@@ -29,6 +29,16 @@
 //            (src/kernels/KernelAB_SOA_Pull.cpp:113-194)
 // AA touches the same Q slots for reading and writing, so it runs in place
 // with no inter-workgroup synchronisation.
+//
+// Interleaved rows (AA on unmasked periodic SOA boxes with nx a multiple of
+// 64, DenseArgs::xk = K = nx / 64): element x of every row is stored at
+// position pos(x) = (x mod K) * 64 + x / K, so the 64 positions 64w .. 64w + 63
+// of block w hold x = w, w + K, ..., w + 63K.  Shifting x by +-1 maps block w
+// onto block w +- 1 lane for lane, and the two end blocks onto each other with
+// the lanes rotated by one (the periodic wrap).  A wavefront that covers one
+// block therefore reads and writes one whole aligned block of each row in the
+// odd step too (K_AA_ODD_IR), exactly as in the even step, which is
+// position-local and unchanged.  Readout and initialisation map x to pos(x).
 #pragma once
 #include "lbm_collide.hpp"
 #include "lbm_internal.hpp"
@@ -37,7 +47,19 @@ namespace lbm {
 
 constexpr uint32_t SOLID_BIT = 1u << 31;
 
-enum DenseKind { K_AA_EVEN = 0, K_AA_ODD = 1, K_AB_PUSH = 2, K_AB_PULL = 3 };
+// K_AA_ODD_IR: the AA odd step on interleaved rows
+enum DenseKind { K_AA_EVEN = 0, K_AA_ODD = 1, K_AB_PUSH = 2, K_AB_PULL = 3, K_AA_ODD_IR = 4 };
+
+// Interleaved rows: position of x, x at position p, and the position of
+// x + d (d in {-1, 0, 1}, periodic) from the position p of x.
+__host__ __device__ __forceinline__ int ir_pos(int x, int K) { return (x % K) * 64 + x / K; }
+__host__ __device__ __forceinline__ int ir_x(int p, int K) { return (p & 63) * K + (p >> 6); }
+__host__ __device__ __forceinline__ int ir_shift(int p, int d, int K) {
+  const int w = p >> 6, l = p & 63;
+  if (d > 0) return w + 1 < K ? p + 64 : ((l + 1) & 63);
+  if (d < 0) return w > 0 ? p - 64 : (K - 1) * 64 + ((l - 1) & 63);
+  return p;
+}
 
 struct DenseArgs {
   int nx, ny, nz;
@@ -53,6 +75,7 @@ struct DenseArgs {
   double djump;                // density jump across the periodic x boundary (0: none)
   int z0;                      // first z-plane of the launch (z = blockIdx.z + z0): 0 except in
                                // the z-slab schedule of the AA steps
+  int xk;                      // interleaved rows: nx / 64; 0: element x at position x
 };
 
 // Pressure-drop boundary: the value a population carries across the periodic
@@ -255,7 +278,7 @@ __device__ __forceinline__ void collide_cell(typename Storage<STORE>::C* d, doub
 template <int Q, int STORE, int KIND, bool FORCE>
 constexpr bool one_collision_body() {
   if (STORE == LBM_STORE_F16S) return true;
-  if (STORE == LBM_STORE_F32S) return KIND == K_AA_ODD && (FORCE || Q == 27);
+  if (STORE == LBM_STORE_F32S) return (KIND == K_AA_ODD || KIND == K_AA_ODD_IR) && (FORCE || Q == 27);
   return false;
 }
 
@@ -280,8 +303,12 @@ template <int Q, int STORE, int LAYOUT, int KIND, bool MASK, bool FORCE, bool NT
 __global__ void __launch_bounds__(kDenseMaxBlock)
     k_dense(const typename Storage<STORE>::T* f0, typename Storage<STORE>::T* f1, DenseArgs a) {
   using C = typename Storage<STORE>::C;
+  constexpr bool IR = KIND == K_AA_ODD_IR;
+  static_assert(!IR || (!MASK && LAYOUT == LBM_LAYOUT_SOA), "interleaved rows: unmasked SOA only");
+  // x: the thread's row position (on interleaved rows, element xr of the row)
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= a.nx) return;
+  const int xr = IR ? ir_x(x, a.xk) : x;
   const int y = blockIdx.y, z = blockIdx.z + a.z0;
   const int64_t cell = x + (int64_t)a.nx * (y + (int64_t)a.ny * z);
   uint32_t lk = 0;
@@ -315,6 +342,12 @@ __global__ void __launch_bounds__(kDenseMaxBlock)
         int ss = s, xx = x, yy = y, zz = z;
         if constexpr (KIND == K_AA_EVEN) {
           ss = o;
+        } else if constexpr (IR) {
+          // the element at x - c_s: a whole aligned block per wavefront
+          xx = ir_shift(x, -cx, a.xk);
+          yy = wrap1(y - cy, a.ny);
+          zz = wrap1(z - cz, a.nz);
+          cross = gather_crossing(cx, xr, a.nx);
         } else if constexpr (KIND == K_AA_ODD || KIND == K_AB_PULL) {
           aligned = cx == 0;
           xx = wrap1(x - cx, a.nx);
@@ -363,6 +396,9 @@ __global__ void __launch_bounds__(kDenseMaxBlock)
       const int cx = Lattice<Q>::c(s, 0), cy = Lattice<Q>::c(s, 1), cz = Lattice<Q>::c(s, 2);
       if constexpr (KIND == K_AA_EVEN || KIND == K_AB_PULL) {
         put_row(row_of(f1, n, s, a, y, z), x, d[s], true);
+      } else if constexpr (IR) {
+        put_row(row_of(f1, n, o, a, wrap1(y + cy, a.ny), wrap1(z + cz, a.nz)), ir_shift(x, cx, a.xk),
+                with_jump<Q>(d[s], scatter_crossing(cx, xr, a.nx), s, a.djump), true);
       } else {
         // streaming target: AA odd writes slot opp(s) at x + c_s, AB push slot s
         put_row(row_of(f1, n, KIND == K_AA_ODD ? o : s, a, wrap1(y + cy, a.ny), wrap1(z + cz, a.nz)),
@@ -833,6 +869,13 @@ DenseLaunchFn pick_kind(int kind, bool mask, bool force) {
     case K_AA_ODD: return pick_flags<Q, STORE, LAYOUT, K_AA_ODD, NT>(mask, force);
     case K_AB_PUSH: return pick_flags<Q, STORE, LAYOUT, K_AB_PUSH, NT>(mask, force);
     case K_AB_PULL: return pick_flags<Q, STORE, LAYOUT, K_AB_PULL, NT>(mask, force);
+    case K_AA_ODD_IR:
+      if constexpr (LAYOUT == LBM_LAYOUT_SOA) {
+        if (mask) return nullptr;
+        return force ? launch_dense<Q, STORE, LAYOUT, K_AA_ODD_IR, false, true, NT>
+                     : launch_dense<Q, STORE, LAYOUT, K_AA_ODD_IR, false, false, NT>;
+      }
+      return nullptr;
   }
   return nullptr;
 }

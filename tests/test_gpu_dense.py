@@ -195,13 +195,14 @@ def oracle_run_jump(geom, q, fc, pattern, steps, omega, force, jump):
 
 @pytest.mark.parametrize("q", [19, 27])
 @pytest.mark.parametrize("pattern", ["aa", "push", "pull"])
-@pytest.mark.parametrize("case", ["box128_staged", "spheres70", "spheres128_staged"])
+@pytest.mark.parametrize("case", ["box128_staged", "box128_irows", "spheres70", "spheres128_staged"])
 def test_dense_pressure_jump_vs_oracle(q, pattern, case, monkeypatch):
     """Pressure-drop boundary (density jump across the periodic x boundary):
     FP64 equals the oracle bit for bit after 7 steps, on a periodic box with
-    rows of 128 (LDS-staged kernels), on a channel with spheres (walls y,
-    unstaged kernels) and on the same channel with rows of 128 and staged
-    mode 2 (the masked staged gather), with a get/set round trip after 4 steps."""
+    rows of 128 (LDS-staged kernels on plain rows; for AA also on interleaved
+    rows, the default there), on a channel with spheres (walls y, unstaged
+    kernels) and on the same channel with rows of 128 and staged mode 2 (the
+    masked staged gather), with a get/set round trip after 4 steps."""
     nx = 70 if case == "spheres70" else 128
     ny, nz = 8, 5
     if case.startswith("box"):
@@ -214,6 +215,7 @@ def test_dense_pressure_jump_vs_oracle(q, pattern, case, monkeypatch):
         solid, periodic = m.astype(np.uint8), (1, 0, 1)
     if case.endswith("staged"):
         monkeypatch.setenv("LBM_DENSE_STAGED", "2")
+    monkeypatch.setenv("LBM_DENSE_IROWS", "1" if case.endswith("irows") else "0")
     geom = O.DenseGeometry(nx, ny, nz, solid, periodic=periodic)
     fc = seeded_fc(geom, q, seed=37)
     omega, force, jump = 1 / 0.7, (1e-6, 0.0, 0.0), 3e-3
@@ -278,11 +280,13 @@ def test_dense_zero_force_kernels(q, pattern, storage, terms, monkeypatch):
 
 @pytest.mark.parametrize("terms", ["0", "1", None])
 @pytest.mark.parametrize("nx,ny,nz", [(128, 6, 5), (512, 3, 2)])
-def test_dense_staged_aa_f64_force_terms_policy(terms, nx, ny, nz, monkeypatch):
+@pytest.mark.parametrize("irows", ["0", "1"])
+def test_dense_staged_aa_f64_force_terms_policy(terms, nx, ny, nz, irows, monkeypatch):
     """D3Q19 FP64 AA on periodic SOA boxes with rows of 128 and 512 (the
-    staged odd step; with 512-cell rows the default runs the kernels with the
-    Guo terms): at F = 0 both kernel sets, and the default choice, equal the
-    oracle bit for bit."""
+    staged odd step on plain rows, the interleaved-row odd step otherwise; with
+    512-cell rows the default runs the kernels with the Guo terms): at F = 0
+    both kernel sets, and the default choice, equal the oracle bit for bit."""
+    monkeypatch.setenv("LBM_DENSE_IROWS", irows)
     if terms is not None:
         monkeypatch.setenv("LBM_DENSE_FORCE_TERMS", terms)
     else:
@@ -311,6 +315,7 @@ def test_dense_staged_rows(q, pattern, storage, force, nt, monkeypatch):
     the oracle bit for bit; FP32/FP16 deviation storage within its precision."""
     monkeypatch.setenv("LBM_DENSE_STAGED", "2")
     monkeypatch.setenv("LBM_DENSE_NT", nt)
+    monkeypatch.setenv("LBM_DENSE_IROWS", "0")
     nx, ny, nz = 128, 6, 5
     geom = O.DenseGeometry(nx, ny, nz)
     fc = seeded_fc(geom, q, seed=12)
@@ -331,10 +336,11 @@ def test_dense_staged_rows(q, pattern, storage, force, nt, monkeypatch):
 
 
 @pytest.mark.parametrize("pattern", ["aa", "push", "pull"])
-def test_dense_staged_wide_rows_d3q27_f64(pattern):
+def test_dense_staged_wide_rows_d3q27_f64(pattern, monkeypatch):
     """D3Q27 FP64 rows of 512 cells stage 18 x 512 doubles (72 KiB) per
     workgroup, above 64 KiB (gfx950 allows 160 KiB): bit for bit with the
     oracle."""
+    monkeypatch.setenv("LBM_DENSE_IROWS", "0")
     nx, ny, nz = 512, 4, 3
     geom = O.DenseGeometry(nx, ny, nz)
     fc = seeded_fc(geom, 27, seed=31)
@@ -527,8 +533,11 @@ def test_dense_aa_slab_schedule(planes, q, storage, masked, monkeypatch):
     geom = O.DenseGeometry(nx, ny, nz, solid, periodic=periodic)
     fc = seeded_fc(geom, q, seed=33)
     omega, F = 1 / 0.6, (1e-6, 0.0, 0.0)
-    for steps in (8, 9):
-        print(f"[dense] slabs of {planes} q={q} {storage} masked={masked} steps={steps}", flush=True)
+    runs = [(steps, irows) for steps in (8, 9) for irows in (("0",) if masked else ("0", "1"))]
+    for steps, irows in runs:
+        monkeypatch.setenv("LBM_DENSE_IROWS", irows)
+        print(f"[dense] slabs of {planes} q={q} {storage} masked={masked} steps={steps} "
+              f"irows={irows}", flush=True)
         with DenseLattice(nx, ny, nz, q=q, pattern="aa", storage=storage, periodic=periodic,
                           omega=omega, force=F, solid=solid) as lat:
             lat.set_state(dense_canonical(geom, q, fc))
@@ -553,6 +562,7 @@ def test_dense_segmented_odd_step(w, q, storage, rows, monkeypatch):
     with a pressure-drop jump across the periodic x boundary.  FP64 equals
     the oracle bit for bit; FP32 / FP16 deviations within 1e-7 / 2e-5."""
     monkeypatch.setenv("LBM_DENSE_ODD_SEG", str(w))
+    monkeypatch.setenv("LBM_DENSE_IROWS", "0")
     nx, (ny, nz) = 256, rows
     geom = O.DenseGeometry(nx, ny, nz)
     fc = seeded_fc(geom, q, seed=44)
@@ -579,6 +589,7 @@ def test_dense_segmented_odd_with_slabs(monkeypatch):
     """Segmented odd step inside the z-slab schedule (z offsets per launch)."""
     monkeypatch.setenv("LBM_DENSE_ODD_SEG", "128")
     monkeypatch.setenv("LBM_DENSE_SLAB_PLANES", "3")
+    monkeypatch.setenv("LBM_DENSE_IROWS", "0")
     nx, ny, nz = 256, 8, 7
     geom = O.DenseGeometry(nx, ny, nz)
     fc = seeded_fc(geom, 19, seed=45)
@@ -588,3 +599,74 @@ def test_dense_segmented_odd_with_slabs(monkeypatch):
         lat.step(9)
         got = lat.get_state()
     assert np.array_equal(got, oracle_run(geom, 19, fc, "aa", 9, 1 / 0.7, (1e-6, 0, 0)))
+
+
+IROWS = [(q, st, nx) for q in (19, 27) for st in ("f64", "f32s", "f16s") for nx in (64, 192, 256)]
+
+
+@pytest.mark.parametrize("q,storage,nx", IROWS)
+@pytest.mark.parametrize("nt", ["0", "1"])
+def test_dense_interleaved_rows(q, storage, nx, nt, monkeypatch):
+    """AA on interleaved rows (element x of a row at position
+    (x mod K) * 64 + x / K, K = nx / 64: the default on unmasked periodic SOA
+    boxes whose rows are whole 64-blocks) for K = 1, 3 and 4, with body force
+    and per-cell tau and gamma fields, cached and streaming: after 7 steps
+    (and a get/set round trip in the odd convention after 5), the state equals
+    the plain-row run bit for bit in every storage, and FP64 equals the oracle
+    bit for bit.  Moments are those of the plain-row run too."""
+    monkeypatch.setenv("LBM_DENSE_NT", nt)
+    ny, nz = 6, 5
+    geom = O.DenseGeometry(nx, ny, nz)
+    fc = seeded_fc(geom, q, seed=52)
+    cells = np.arange(nx * ny * nz, dtype=np.uint64)
+    tau = 0.55 + 0.4 * O.uniform01(61, cells)
+    gamma = 0.3 * O.uniform01(62, cells)
+    omega, F = 1 / 0.6, (1e-5, -2e-6, 3e-6)
+    got = {}
+    for irows in ("0", "1"):
+        monkeypatch.setenv("LBM_DENSE_IROWS", irows)
+        print(f"[dense] interleaved rows={irows} q={q} {storage} nx={nx} nt={nt}", flush=True)
+        for fields in (False, True):
+            with DenseLattice(nx, ny, nz, q=q, pattern="aa", storage=storage, omega=omega, force=F,
+                              tau=tau if fields else None, gamma=gamma if fields else None) as lat:
+                lat.set_state(fc)
+                lat.step(5)
+                lat.set_state(lat.get_state())  # t = 0 again: the even convention
+                lat.step(2)
+                lat.step(1)
+                got[irows, fields] = (lat.get_state(), lat.moments())
+    for fields in (False, True):
+        a, b = got["0", fields], got["1", fields]
+        assert np.array_equal(a[0], b[0]), fields
+        assert np.array_equal(a[1][0], b[1][0]) and np.array_equal(a[1][1], b[1][1]), fields
+    if storage == "f64":
+        assert np.array_equal(got["1", False][0], oracle_run(geom, q, fc, "aa", 8, omega, F))
+        assert np.array_equal(got["1", True][0],
+                              oracle_run(geom, q, fc, "aa", 8, omega, F, tau=tau, gamma=gamma))
+
+
+@pytest.mark.parametrize("storage", ["f64", "f16s"])
+def test_dense_interleaved_rows_init_and_odd_readout(storage, monkeypatch):
+    """On interleaved rows (rows of 128): the Taylor-Green initial state, the
+    state read after an odd step count, the run continued from that state
+    set again, and the moments and mass of it equal those of plain rows bit
+    for bit."""
+    nx, ny, nz = 128, 8, 6
+    res = {}
+    for irows in ("0", "1"):
+        monkeypatch.setenv("LBM_DENSE_IROWS", irows)
+        with DenseLattice(nx, ny, nz, q=19, pattern="aa", storage=storage, omega=1 / 0.6) as lat:
+            lat.init_tgv(0.05, 1e-4, 5)
+            s0 = lat.get_state()
+            lat.step(3)
+            s3 = lat.get_state()
+            lat.set_state(s3)
+            lat.step(3)
+            rt = lat.get_state()
+            lat.set_state(s3)
+            res[irows] = (s0, s3, lat.moments(), lat.mass(), rt)
+    a, b = res["0"], res["1"]
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert np.array_equal(a[2][0], b[2][0]) and np.array_equal(a[2][1], b[2][1])
+    assert a[3] == b[3]
+    assert np.array_equal(a[4], b[4])
