@@ -339,13 +339,17 @@ size_t lds_for_blocks_per_cu(int k) {
 // step, best of a sweep over widths 64-512 and 0-8 workgroups per CU:
 //   D3Q19 FP64   128 wide, <= 4 per CU: 6.91 ms (512-wide rows: 7.86 ms)
 //   D3Q19 FP32   256 wide, <= 4 per CU: 3.39 ms (256 wide, no cap: 3.47 ms)
-//   D3Q19 FP16   256 wide, no cap:      1.85 ms (512-wide rows: 2.06 ms)
+//   D3Q19 FP16   512 wide, <= 4 per CU: 1.74 ms per step, even and odd
+//                averaged over 100-step runs with the interleaved-row odd
+//                step (256 wide, no cap: 1.76-1.78; 128 wide: 1.84-1.85)
 //   D3Q27 FP64   256 wide, <= 2 per CU: 9.31 ms (512-wide rows: 10.83 ms)
 //   D3Q27 FP32   128 wide, no cap:      4.69 ms (256 wide, no cap: 4.86 ms)
-//   D3Q27 FP16   128 wide, <= 8 per CU: 2.49 ms (512-wide rows: 2.70 ms)  Environment overrides (tuning): LBM_DENSE_BLOCK_X
-// and LBM_DENSE_BLOCKS_PER_CU for the unstaged kernels (the latter also for
-// the staged ones), LBM_DENSE_EVEN_BLOCK_X and LBM_DENSE_EVEN_BLOCKS_PER_CU for
-// the even step.
+//   D3Q27 FP16   128 wide, <= 8 per CU: 2.49 ms (512-wide rows: 2.70 ms)
+// FP64 checked again in 100-step runs: 128 wide, <= 4 per CU 7.00-7.08 ms per
+// step; 128 / 2: 7.05-7.09; 64 / 4: 7.10-7.13; wider or more per CU slower.
+// Environment overrides (tuning): LBM_DENSE_BLOCK_X and LBM_DENSE_BLOCKS_PER_CU
+// for the unstaged kernels (the latter also for the staged ones),
+// LBM_DENSE_EVEN_BLOCK_X and LBM_DENSE_EVEN_BLOCKS_PER_CU for the even step.
 void set_launch_shapes(lbm_dense* h) {
   auto env_int = [](const char* name, int dflt) {
     const char* e = std::getenv(name);
@@ -358,7 +362,7 @@ void set_launch_shapes(lbm_dense* h) {
   if (h->layout == LBM_LAYOUT_SOA) {
     static const int shape[2][3][2] = {
         // {width, workgroups per CU} for F64, F32S, F16S
-        {{128, 4}, {256, 4}, {256, 0}},  // D3Q19
+        {{128, 4}, {256, 4}, {512, 4}},  // D3Q19
         {{256, 2}, {128, 0}, {128, 8}},  // D3Q27
     };
     const int* sh = shape[h->q == 19 ? 0 : 1][h->storage];
