@@ -211,8 +211,8 @@ struct lbm_sparse {
   double force[3] = {1e-6, 0.0, 0.0};
   int32_t* adj = nullptr;  // device copy of the adjacency
   // launch shapes: the AA even step (all node-aligned) and the other steps
-  unsigned even_bx = 256, step_bx = 256;
-  size_t even_lds = 0, step_lds = 0;
+  unsigned even_bx = 256, step_bx = 256, push_bx = 256;
+  size_t even_lds = 0, step_lds = 0, push_lds = 0;
   bool nt = true;  // non-temporal node-aligned accesses (arrays beyond the memory-side cache)
   hipStream_t stream = nullptr;  // null stream unless the caller set one
   int xcd_env = -1;              // LBM_SPARSE_XCD: node blocks per XCD run (-1: by footprint)
@@ -271,11 +271,14 @@ void launch_qt(const lbm_sparse* h, Shape sh, A... a) {
 //   streaming (simulationSize 8, 1.35 GB): runs of 8-32 blocks raise AB push
 //   10.6k -> 11.2k, while AA and pull lost with long runs (14.7k -> 13.3k,
 //   13.4k -> 11.4k with one run per XCD) and gained nothing with short ones.
+//   With the push's 128-wide capped blocks (set_shapes), runs of 64-128
+//   blocks measured best: 11.46-11.47k (16: 11.16-11.20k; 256: 11.34k).
 // LBM_SPARSE_XCD (read at create) overrides: 0 off, g > 0 runs of g blocks.
 uint32_t xcd_group(const lbm_sparse* h, bool push, unsigned nb) {
   if (h->xcd_env >= 0) return (uint32_t)h->xcd_env;
   if (!h->nt) return (nb + 7) / 8;
-  return push ? 16u : 0u;
+  if (!push) return 0u;
+  return h->push_bx == 128 ? 64u : 16u;
 }
 
 Shape make_shape(const lbm_sparse* h, int64_t count, unsigned bx, size_t lds, bool push) {
@@ -289,14 +292,19 @@ Shape even_shape(const lbm_sparse* h, int64_t count) {
   return make_shape(h, count, h->even_bx, h->even_lds, false);
 }
 Shape step_shape(const lbm_sparse* h, int64_t count, bool push = false) {
-  return make_shape(h, count, h->step_bx, h->step_lds, push);
+  return push ? make_shape(h, count, h->push_bx, h->push_lds, true)
+              : make_shape(h, count, h->step_bx, h->step_lds, false);
 }
 
 // Launch shapes, fixed at create.  The AA even step is all node-aligned, like
 // the dense engine's even step, which measured fastest with small workgroups
-// at capped occupancy; the other steps keep 256-wide workgroups.  Environment
+// at capped occupancy.  So did the AB push on streaming FP64 arrays
+// (simulationSize-8 deck, MLUPS: 128 wide, <= 4 per CU with XCD runs of 64,
+// 11.46k; 256 wide, no cap, runs of 16: 11.18-11.21k; 64 wide, <= 8 per CU:
+// 10.9-11.3k).  The other steps keep 256-wide workgroups.  Environment
 // overrides (tuning): LBM_SPARSE_EVEN_BLOCK_X, LBM_SPARSE_EVEN_BLOCKS_PER_CU,
-// LBM_SPARSE_BLOCK_X, LBM_SPARSE_BLOCKS_PER_CU.
+// LBM_SPARSE_BLOCK_X, LBM_SPARSE_BLOCKS_PER_CU (all other steps, the push
+// included).
 void set_shapes(lbm_sparse* h) {
   auto env_int = [](const char* name, int dflt) {
     const char* e = std::getenv(name);
@@ -318,6 +326,8 @@ void set_shapes(lbm_sparse* h) {
   h->even_lds = lds_for(env_int("LBM_SPARSE_EVEN_BLOCKS_PER_CU", big ? 4 : 0));
   h->step_bx = clamp_bx(env_int("LBM_SPARSE_BLOCK_X", 256));
   h->step_lds = lds_for(env_int("LBM_SPARSE_BLOCKS_PER_CU", 0));
+  h->push_bx = clamp_bx(env_int("LBM_SPARSE_BLOCK_X", big ? 128 : 256));
+  h->push_lds = lds_for(env_int("LBM_SPARSE_BLOCKS_PER_CU", big ? 4 : 0));
 }
 
 lbm_status check_range(const lbm_sparse* h, int64_t start, int64_t count) {
