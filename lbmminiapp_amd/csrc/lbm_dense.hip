@@ -649,12 +649,16 @@ lbm_status dense_create(const lbm_dense_config* cfg, lbm_dense** out, bool push_
   // Interleaved rows (lbm_dense_kernels.hpp) for AA on unmasked periodic SOA
   // boxes whose rows are whole 64-element blocks: the odd step then makes only
   // whole aligned block accesses, like the even step.  Per-cell fields are
-  // stored in the same order.  Default everywhere except D3Q27 FP32
-  // deviations, where plain rows measured faster (set_launch_shapes).
-  // LBM_DENSE_IROWS=0|1 overrides (A/B timing).
+  // stored in the same order.  Default for grids that stream from HBM (h->nt),
+  // except D3Q27 FP32 deviations, where plain rows measured faster
+  // (set_launch_shapes).  Grids that fit the memory-side cache gained nothing
+  // (D3Q19 100-400-step runs, MLUPS interleaved / plain rows: 64^3 FP64 18.7k
+  // uncapped, 16.6k with the streaming shape / 19.1-19.5k; 128^3 FP16 65.8-67.3k
+  // / 66.4-67.8k), while streaming ones did (256^3 FP16 73.0k / 68.0-68.6k;
+  // 256^3 FP64 18.7k / 18.5k).  LBM_DENSE_IROWS=0|1 overrides (A/B timing).
   {
     const char* e = std::getenv("LBM_DENSE_IROWS");
-    const bool on = e ? e[0] != '0' : !(h->q == 27 && h->storage == LBM_STORE_F32S);
+    const bool on = e ? e[0] != '0' : h->nt && !(h->q == 27 && h->storage == LBM_STORE_F32S);
     if (on && h->pattern == LBM_PATTERN_AA && h->layout == LBM_LAYOUT_SOA && !need_links &&
         h->nx % 64 == 0) {
       h->xk = h->nx / 64;
