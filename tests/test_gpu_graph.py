@@ -16,8 +16,11 @@ pytestmark = pytest.mark.gpu
 NX, NY, NZ = 64, 32, 24
 
 CASES = {
-    # periodic FP64 AA: the LDS-staged odd kernel
-    "aa_f64_staged": dict(pattern="aa", storage="f64"),
+    # periodic FP64 AA: the LDS-staged odd kernel on plain rows, and the odd
+    # step on interleaved rows
+    "aa_f64_staged": dict(pattern="aa", storage="f64", irows="0"),
+    "aa_f64_irows": dict(pattern="aa", storage="f64", irows="1", force=(1e-5, 0, 0)),
+    "aa_f16s_irows_jump": dict(pattern="aa", storage="f16s", irows="1", jump=2e-4),
     "aa_f32s_masked_force": dict(pattern="aa", storage="f32s", solid=True, force=(1e-5, 0, 0)),
     "ab_pull_f16s_masked": dict(pattern="ab", pull=True, storage="f16s", solid=True),
     "ab_push_f64_masked": dict(pattern="ab", pull=False, storage="f64", solid=True, force=(0, 1e-5, 0)),
@@ -95,6 +98,8 @@ def test_graph_replays_equal_eager_steps(name, monkeypatch):
     case = CASES[name]
     if "env" in case:
         monkeypatch.setenv("LBM_DENSE_PUSH_GATHER", case["env"])
+    if "irows" in case:
+        monkeypatch.setenv("LBM_DENSE_IROWS", case["irows"])
     print(f"[graph] {name} {NX}x{NY}x{NZ}", flush=True)
     stream = _Stream()
     try:
