@@ -16,6 +16,9 @@ by device synchronisation on both sides.  roofline.achieved = 2 Q b bytes per
 node * nodes / mean kernel duration from HIP events recorded on the engine's
 stream around the same K launches.
 
+copy_kernel: the measured copy ceiling of the same GPU (a 16-byte-per-lane
+copy of 4 GiB), and the fraction of it the headline leg reaches.
+
 cpu_baseline: the reference's own AA kernel (KernelAA_SOA_Unroll, the
 `unroll 1` kernel of its benchmark decks) compiled from its sources into
 oracle/_ref/, one process per host core (the reference's one MPI rank per
@@ -131,6 +134,19 @@ def adjoint_leg(n=128, steps=500, every=50, storage="f64"):
             "objective": J, "device_bytes": mem,
             "config": f"D3Q19 {n}^3 channel, Brinkman gamma field, {steps} steps, "
                       f"checkpoint every {every}, {storage}"}
+
+
+def copy_peak(nbytes=4 << 30, reps=10):
+    """The measured copy ceiling of this GPU: the library's 16-byte copy kernel
+    between two nbytes buffers (nbytes read + nbytes written per dispatch,
+    reps timed dispatches after one untimed)."""
+    from lbmminiapp_amd import capi
+
+    gbs = capi.copy_bandwidth(nbytes, reps)
+    return {"value": round(gbs, 1), "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 4),
+            "bytes_per_dispatch": 2 * nbytes,
+            "note": "16-byte-per-lane copy kernel, 4 GiB read + 4 GiB written per dispatch; "
+                    "frac_reached = the headline leg's achieved bandwidth / this"}
 
 
 SLAB_PLANES = 8  # z-planes of 128 x 128 cells per reference rank
@@ -287,6 +303,9 @@ def main():
               f"{results[leg]['achieved_gbs']:.0f} GB/s ({100 * results[leg]['frac']:.1f}% of "
               f"{PEAK_HBM_GBS:.0f} GB/s)", file=sys.stderr, flush=True)
     head = results.get("f64") or results[legs[0]]
+    copy = copy_peak()
+    print(f"# copy kernel: {copy['value']:.0f} GB/s ({100 * copy['frac']:.1f}% of "
+          f"{PEAK_HBM_GBS:.0f} GB/s)", file=sys.stderr, flush=True)
     workload = f"D3Q{q} AA SOA {n}^3 periodic Taylor-Green, tau={TAU}, {head['storage']}"
     traffic = load_traffic(f"d3q{q}_aa_{n}_{head['storage']}")
     line = {
@@ -312,6 +331,7 @@ def main():
                     "odd_ms": round(head["odd_kernel_ms"], 4),
                     "bytes_per_lup": head["bytes_per_lup"]},
         "mass_drift": head["mass_drift"],
+        "copy_kernel": dict(copy, frac_reached=round(head["achieved_gbs"] / copy["value"], 4)),
     }
     for leg in legs:
         if results[leg] is head:

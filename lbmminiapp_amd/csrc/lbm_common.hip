@@ -79,26 +79,15 @@ lbm_status lbm_device_synchronize(void) {
   return LBM_OK;
 }
 
-lbm_status lbm_copy_bandwidth(size_t bytes, int32_t reps, double* elapsed_ms) {
-  LBM_REQUIRE(elapsed_ms && reps > 0 && bytes >= 16 && bytes % 16 == 0, LBM_ERR_ARG,
-              "lbm_copy_bandwidth: bytes must be a positive multiple of 16, reps > 0");
-  void *a = nullptr, *b = nullptr;
-  LBM_HIP_CHECK(hipMalloc(&a, bytes));
-  if (hipMalloc(&b, bytes) != hipSuccess) {
-    (void)hipFree(a);
-    lbm::set_error("lbm_copy_bandwidth: hipMalloc of %zu bytes failed", bytes);
-    return LBM_ERR_OOM;
-  }
-  hipStream_t st;
-  LBM_HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-  LBM_HIP_CHECK(hipMemsetAsync(a, 0, bytes, st));
+namespace {
+// The timed copy of lbm_copy_bandwidth on resources the caller owns and frees.
+lbm_status copy_timed(const void* a, void* b, size_t bytes, int32_t reps, hipStream_t st,
+                      hipEvent_t e0, hipEvent_t e1, double* elapsed_ms) {
+  LBM_HIP_CHECK(hipMemsetAsync(const_cast<void*>(a), 0, bytes, st));
   const size_t n16 = bytes / 16;
   const int block = 256;
   size_t grid = (n16 + block - 1) / block;
   if (grid > 1048576) grid = 1048576;
-  hipEvent_t e0, e1;
-  LBM_HIP_CHECK(hipEventCreate(&e0));
-  LBM_HIP_CHECK(hipEventCreate(&e1));
   // one untimed launch to page in both buffers
   lbm::k_copy16<<<dim3((unsigned)grid), dim3(block), 0, st>>>((const uint4*)a, (uint4*)b, n16);
   LBM_LAUNCH_CHECK();
@@ -111,12 +100,34 @@ lbm_status lbm_copy_bandwidth(size_t bytes, int32_t reps, double* elapsed_ms) {
   float ms = 0.f;
   LBM_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
   *elapsed_ms = ms;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  (void)hipStreamDestroy(st);
-  (void)hipFree(a);
-  (void)hipFree(b);
   return LBM_OK;
+}
+}  // namespace
+
+lbm_status lbm_copy_bandwidth(size_t bytes, int32_t reps, double* elapsed_ms) {
+  LBM_REQUIRE(elapsed_ms && reps > 0 && bytes >= 16 && bytes % 16 == 0, LBM_ERR_ARG,
+              "lbm_copy_bandwidth: bytes must be a positive multiple of 16, reps > 0");
+  void *a = nullptr, *b = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  lbm_status status = LBM_OK;
+  if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
+    lbm::set_error("lbm_copy_bandwidth: hipMalloc of 2 x %zu bytes failed", bytes);
+    status = LBM_ERR_OOM;
+  } else if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+             hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    lbm::set_error("lbm_copy_bandwidth: stream/event creation failed");
+    status = LBM_ERR_HIP;
+  } else {
+    status = copy_timed(a, b, bytes, reps, st, e0, e1, elapsed_ms);
+  }
+  if (st) (void)hipStreamSynchronize(st);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  return status;
 }
 
 }  // extern "C"
