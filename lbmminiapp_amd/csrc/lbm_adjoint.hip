@@ -521,7 +521,7 @@ lbm_status lbm_adjoint_create(const lbm_adjoint_config* cfg, lbm_adjoint** out) 
   dc.tau = cfg->tau;
   dc.gamma = cfg->gamma;
   lbm_adjoint* h = new lbm_adjoint;
-  if (lbm_status st = dense_create(&dc, &h->d, false)) {
+  if (lbm_status st = dense_create(&dc, &h->d, false, false)) {
     delete h;
     return st;
   }

@@ -56,7 +56,7 @@ __device__ __forceinline__ bool cell_coords(const DenseArgs& a, int& x, int& y, 
 template <int Q>
 __global__ void __launch_bounds__(256)
     k_build_links(const uint8_t* __restrict__ solid, uint32_t* __restrict__ links, int nx, int ny,
-                  int nz, int px, int py, int pz) {
+                  int nz, int px, int py, int pz, int xk) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y, z = blockIdx.z;
   if (x >= nx) return;
@@ -78,7 +78,8 @@ __global__ void __launch_bounds__(256)
     if (!wall) wall = solid[p[0] + (int64_t)nx * (p[1] + (int64_t)ny * p[2])] != 0;
     if (wall) m |= 1u << s;
   }
-  links[cell] = m;
+  // stored like the populations: at the cell's position on interleaved rows
+  links[xk ? cell - x + ir_pos(x, xk) : cell] = m;
 }
 
 // ---------------------------------------------------------------- state in / out
@@ -99,7 +100,7 @@ __global__ void __launch_bounds__(256)
   int x, y, z;
   int64_t cell;
   if (!cell_coords(a, x, y, z, cell)) return;
-  const uint32_t lk = a.links ? a.links[cell] : 0u;
+  const uint32_t lk = a.links ? a.links[store_cell(a, cell, x)] : 0u;
   if (lk & SOLID_BIT) return;
   const Offsets off(x, y, z, a.nx, a.ny, a.nz);
   for (int s = 0; s < Q; s++) {
@@ -124,7 +125,7 @@ __global__ void __launch_bounds__(256)
   int x, y, z;
   int64_t cell;
   if (!cell_coords(a, x, y, z, cell)) return;
-  const uint32_t lk = a.links ? a.links[cell] : 0u;
+  const uint32_t lk = a.links ? a.links[store_cell(a, cell, x)] : 0u;
   const Offsets off(x, y, z, a.nx, a.ny, a.nz);
   for (int s = 0; s < Q; s++) {
     double v = 0.0;
@@ -148,7 +149,7 @@ __global__ void __launch_bounds__(256)
   int x, y, z;
   int64_t cell;
   if (!cell_coords(a, x, y, z, cell)) return;
-  const uint32_t lk = a.links ? a.links[cell] : 0u;
+  const uint32_t lk = a.links ? a.links[store_cell(a, cell, x)] : 0u;
   double rho = 0.0, m0 = 0.0, m1 = 0.0, m2 = 0.0;
   if (!(lk & SOLID_BIT)) {
     const Offsets off(x, y, z, a.nx, a.ny, a.nz);
@@ -197,7 +198,7 @@ __global__ void __launch_bounds__(256)
   int x, y, z;
   int64_t cell;
   if (!cell_coords(a, x, y, z, cell)) return;
-  const uint32_t lk = a.links ? a.links[cell] : 0u;
+  const uint32_t lk = a.links ? a.links[store_cell(a, cell, x)] : 0u;
   if (lk & SOLID_BIT) return;
   double rho = 1.0, u[3] = {0.0, 0.0, 0.0};
   if (src_kind == 1) {
@@ -483,7 +484,7 @@ lbm_status launch_kind(lbm_dense* h, int kind, void* f0, void* f1, int z0, int n
   }
   DenseArgs a = h->args;
   a.z0 = z0;
-  if (kind == K_AA_ODD && h->xk) {
+  if (kind == K_AA_ODD && h->xk && !h->links) {
     DenseLaunchFn fn = pick_step(h, K_AA_ODD_IR);
     LBM_REQUIRE(fn, LBM_ERR_UNSUPPORTED, "no interleaved-row odd kernel for this configuration");
     const dim3 b(h->odd_ir_bx);
@@ -601,7 +602,8 @@ lbm_status init_from(lbm_dense* h, int src_kind, const double* rho_d, const doub
   return LBM_OK;
 }
 
-lbm_status dense_create(const lbm_dense_config* cfg, lbm_dense** out, bool push_by_gather) {
+lbm_status dense_create(const lbm_dense_config* cfg, lbm_dense** out, bool push_by_gather,
+                        bool irows_ok) {
   LBM_REQUIRE(cfg && out, LBM_ERR_ARG, "lbm_dense_create: null pointer");
   *out = nullptr;
   LBM_REQUIRE(cfg->nx > 0 && cfg->ny > 0 && cfg->nz > 0, LBM_ERR_ARG,
@@ -658,9 +660,10 @@ lbm_status dense_create(const lbm_dense_config* cfg, lbm_dense** out, bool push_
   // 256^3 FP64 18.7k / 18.5k).  LBM_DENSE_IROWS=0|1 overrides (A/B timing).
   {
     const char* e = std::getenv("LBM_DENSE_IROWS");
-    const bool on = e ? e[0] != '0' : h->nt && !(h->q == 27 && h->storage == LBM_STORE_F32S);
-    if (on && h->pattern == LBM_PATTERN_AA && h->layout == LBM_LAYOUT_SOA && !need_links &&
-        h->nx % 64 == 0) {
+    const bool dflt = h->pattern == LBM_PATTERN_AA && !need_links && h->nt &&
+                      !(h->q == 27 && h->storage == LBM_STORE_F32S);
+    const bool on = e ? e[0] != '0' : dflt;
+    if (on && irows_ok && h->layout == LBM_LAYOUT_SOA && h->nx % 64 == 0) {
       h->xk = h->nx / 64;
       h->odd_seg_w = 0;
     }
@@ -723,10 +726,10 @@ lbm_status dense_create(const lbm_dense_config* cfg, lbm_dense** out, bool push_
       const dim3 g = grid_for(h), b = block_for(h->nx);
       if (h->q == 19)
         hipLaunchKernelGGL(k_build_links<19>, g, b, 0, h->stream, solid_d, h->links, h->nx,
-                           h->ny, h->nz, h->periodic[0], h->periodic[1], h->periodic[2]);
+                           h->ny, h->nz, h->periodic[0], h->periodic[1], h->periodic[2], h->xk);
       else
         hipLaunchKernelGGL(k_build_links<27>, g, b, 0, h->stream, solid_d, h->links, h->nx,
-                           h->ny, h->nz, h->periodic[0], h->periodic[1], h->periodic[2]);
+                           h->ny, h->nz, h->periodic[0], h->periodic[1], h->periodic[2], h->xk);
       e = hipGetLastError();
       if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     }
@@ -829,7 +832,7 @@ lbm_status dense_create(const lbm_dense_config* cfg, lbm_dense** out, bool push_
 extern "C" {
 
 lbm_status lbm_dense_create(const lbm_dense_config* cfg, lbm_dense** out) {
-  return dense_create(cfg, out, true);
+  return dense_create(cfg, out, true, true);
 }
 
 lbm_status lbm_dense_destroy(lbm_dense* h) {

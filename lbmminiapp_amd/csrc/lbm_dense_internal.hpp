@@ -51,8 +51,10 @@ size_t lds_for_blocks_per_cu(int k);
 // device allocation accounted in h->device_bytes
 lbm_status dmalloc(lbm_dense* h, void** p, size_t bytes, const char* what);
 // lbm_dense_create; push_by_gather = false keeps AB push handles on the
-// scatter kernel and the push storage convention (the adjoint drives the
-// push kernel on the handle's arrays itself)
-lbm_status dense_create(const lbm_dense_config* cfg, lbm_dense** out, bool push_by_gather);
+// scatter kernel and the push storage convention, irows_ok = false keeps plain
+// rows (the adjoint drives the push kernel on the handle's arrays itself and
+// indexes them by cell)
+lbm_status dense_create(const lbm_dense_config* cfg, lbm_dense** out, bool push_by_gather,
+                        bool irows_ok);
 
 }  // namespace lbm

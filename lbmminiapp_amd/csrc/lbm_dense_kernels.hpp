@@ -30,15 +30,19 @@
 // AA touches the same Q slots for reading and writing, so it runs in place
 // with no inter-workgroup synchronisation.
 //
-// Interleaved rows (AA on unmasked periodic SOA boxes with nx a multiple of
-// 64, DenseArgs::xk = K = nx / 64): element x of every row is stored at
-// position pos(x) = (x mod K) * 64 + x / K, so the 64 positions 64w .. 64w + 63
-// of block w hold x = w, w + K, ..., w + 63K.  Shifting x by +-1 maps block w
-// onto block w +- 1 lane for lane, and the two end blocks onto each other with
-// the lanes rotated by one (the periodic wrap).  A wavefront that covers one
-// block therefore reads and writes one whole aligned block of each row in the
-// odd step too (K_AA_ODD_IR), exactly as in the even step, which is
-// position-local and unchanged.  Readout and initialisation map x to pos(x).
+// Interleaved rows (SOA boxes with nx a multiple of 64, DenseArgs::xk = K =
+// nx / 64): element x of every row is stored at position
+// pos(x) = (x mod K) * 64 + x / K, so the 64 positions 64w .. 64w + 63 of block
+// w hold x = w, w + K, ..., w + 63K.  Shifting x by +-1 maps block w onto block
+// w +- 1 lane for lane, and the two end blocks onto each other with the lanes
+// rotated by one (the periodic wrap).  A wavefront that covers one block
+// therefore reads and writes one whole aligned block of each row in the AA odd
+// step too (K_AA_ODD_IR on unmasked boxes), exactly as in the even step, which
+// is position-local and unchanged.  The other kinds take interleaved rows at
+// run time (a.xk): their x-neighbour offsets come from ir_shift, and the tests
+// that need the coordinate (walls, pressure-drop crossings) use x itself.  The
+// link mask and the per-cell fields are stored in the same order.  Readout and
+// initialisation map x to pos(x).
 #pragma once
 #include "lbm_collide.hpp"
 #include "lbm_internal.hpp"
@@ -305,10 +309,13 @@ __global__ void __launch_bounds__(kDenseMaxBlock)
   using C = typename Storage<STORE>::C;
   constexpr bool IR = KIND == K_AA_ODD_IR;
   static_assert(!IR || (!MASK && LAYOUT == LBM_LAYOUT_SOA), "interleaved rows: unmasked SOA only");
-  // x: the thread's row position (on interleaved rows, element xr of the row)
+  // x: the thread's row position (on interleaved rows, a.xk != 0, element xr
+  // of the row).  Interleaved rows make every x-shifted access one aligned
+  // block as well (see the file comment).
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= a.nx) return;
-  const int xr = IR ? ir_x(x, a.xk) : x;
+  const int xr = a.xk ? ir_x(x, a.xk) : x;
+  const bool irows = IR || a.xk != 0;
   const int y = blockIdx.y, z = blockIdx.z + a.z0;
   const int64_t cell = x + (int64_t)a.nx * (y + (int64_t)a.ny * z);
   uint32_t lk = 0;
@@ -322,7 +329,11 @@ __global__ void __launch_bounds__(kDenseMaxBlock)
     const bool all_solid = __all(solid);
     if (solid && (!a.fill_solid || all_solid)) return;
   }
-  const Offsets off(x, y, z, a.nx, a.ny, a.nz);
+  Offsets off(xr, y, z, a.nx, a.ny, a.nz);
+  if (a.xk) {
+    off.ox[0] = ir_shift(x, -1, a.xk) - x;
+    off.ox[2] = ir_shift(x, 1, a.xk) - x;
+  }
   const int64_t n = a.ncell;
 
   // Fully periodic SOA boxes (no mask): every access is a row element, see row_of.
@@ -349,11 +360,11 @@ __global__ void __launch_bounds__(kDenseMaxBlock)
           zz = wrap1(z - cz, a.nz);
           cross = gather_crossing(cx, xr, a.nx);
         } else if constexpr (KIND == K_AA_ODD || KIND == K_AB_PULL) {
-          aligned = cx == 0;
-          xx = wrap1(x - cx, a.nx);
+          aligned = cx == 0 || irows;
+          xx = irows ? ir_shift(x, -cx, a.xk) : wrap1(x - cx, a.nx);
           yy = wrap1(y - cy, a.ny);
           zz = wrap1(z - cz, a.nz);
-          cross = gather_crossing(cx, x, a.nx);
+          cross = gather_crossing(cx, xr, a.nx);
         }
         const auto* rp = row_of(f0, n, ss, a, yy, zz);
         if (aligned) d[s] = ld_row<STORE, NT>(rp, (uint32_t)xx, a.inv_scale);
@@ -367,12 +378,12 @@ __global__ void __launch_bounds__(kDenseMaxBlock)
       } else if constexpr (KIND == K_AB_PUSH) {
         idx = slot<Q, LAYOUT>(n, s, cell);
       } else {
-        aligned = cx == 0;
+        aligned = cx == 0 || irows;
         if (MASK && ((lk >> o) & 1u)) {
           idx = slot<Q, LAYOUT>(n, o, cell);
         } else {
           idx = slot<Q, LAYOUT>(n, s, cell + off.at(-cx, -cy, -cz));
-          cross = gather_crossing(cx, x, a.nx);
+          cross = gather_crossing(cx, xr, a.nx);
         }
       }
       if (aligned && LAYOUT == LBM_LAYOUT_SOA) d[s] = ld_s<STORE, NT>(f0, idx, a.inv_scale);
@@ -402,8 +413,8 @@ __global__ void __launch_bounds__(kDenseMaxBlock)
       } else {
         // streaming target: AA odd writes slot opp(s) at x + c_s, AB push slot s
         put_row(row_of(f1, n, KIND == K_AA_ODD ? o : s, a, wrap1(y + cy, a.ny), wrap1(z + cz, a.nz)),
-                wrap1(x + cx, a.nx), with_jump<Q>(d[s], scatter_crossing(cx, x, a.nx), s, a.djump),
-                cx == 0);
+                irows ? ir_shift(x, cx, a.xk) : wrap1(x + cx, a.nx),
+                with_jump<Q>(d[s], scatter_crossing(cx, xr, a.nx), s, a.djump), cx == 0 || irows);
       }
     }
     return;
@@ -419,16 +430,16 @@ __global__ void __launch_bounds__(kDenseMaxBlock)
     if constexpr (KIND == K_AA_EVEN || KIND == K_AB_PULL) {
       put(slot<Q, LAYOUT>(n, s, cell), solid ? C(0) : d[s], true);
     } else {
-      const bool aligned = cx == 0;
+      const bool aligned = cx == 0 || irows;
       // streaming target: AA odd writes slot opp(s) at x + c_s, AB push slot s
       const int64_t sidx = slot<Q, LAYOUT>(n, KIND == K_AA_ODD ? o : s, cell + off.at(cx, cy, cz));
       const bool bb = MASK && ((lk >> s) & 1u);
       if (!bb) {
-        if (!solid) put(sidx, with_jump<Q>(d[s], scatter_crossing(cx, x, a.nx), s, a.djump), aligned);
+        if (!solid) put(sidx, with_jump<Q>(d[s], scatter_crossing(cx, xr, a.nx), s, a.djump), aligned);
       } else {
         // halfway bounce-back slot: AA odd slot s at x, AB push slot opp(s) at x
         const int64_t bidx = slot<Q, LAYOUT>(n, KIND == K_AA_ODD ? s : o, cell);
-        const bool out = leaves_box(a, x, y, z, cx, cy, cz);
+        const bool out = leaves_box(a, xr, y, z, cx, cy, cz);
         if (!solid) put(bidx, d[s], aligned);
         else if (out) put(bidx, C(0), aligned);
         if (!out && a.fill_solid) put(sidx, C(0), aligned);
