@@ -345,7 +345,9 @@ size_t lds_for_blocks_per_cu(int k) {
 //                step (256 wide, no cap: 1.76-1.78; 128 wide: 1.84-1.85)
 //   D3Q27 FP64   256 wide, <= 2 per CU: 9.31 ms (512-wide rows: 10.83 ms)
 //   D3Q27 FP32   128 wide, no cap:      4.69 ms (256 wide, no cap: 4.86 ms)
-//   D3Q27 FP16   128 wide, <= 8 per CU: 2.49 ms (512-wide rows: 2.70 ms)
+//   D3Q27 FP16   512 wide, <= 4 per CU: 2.49-2.50 ms per step, even and odd
+//                averaged over 100-step runs with the interleaved-row odd
+//                step (128 wide, <= 8 per CU: 2.61-2.67; 256 / 4: 2.53)
 // FP64 checked again in 100-step runs: 128 wide, <= 4 per CU 7.00-7.08 ms per
 // step; 128 / 2: 7.05-7.09; 64 / 4: 7.10-7.13; wider or more per CU slower.
 // Environment overrides (tuning): LBM_DENSE_BLOCK_X and LBM_DENSE_BLOCKS_PER_CU
@@ -364,7 +366,7 @@ void set_launch_shapes(lbm_dense* h) {
     static const int shape[2][3][2] = {
         // {width, workgroups per CU} for F64, F32S, F16S
         {{128, 4}, {256, 4}, {512, 4}},  // D3Q19
-        {{256, 2}, {128, 0}, {128, 8}},  // D3Q27
+        {{256, 2}, {128, 0}, {512, 4}},  // D3Q27
     };
     const int* sh = shape[h->q == 19 ? 0 : 1][h->storage];
     even_bx = sh[0];
