@@ -182,7 +182,9 @@ lbm_status lbm_dense_create(const lbm_dense_config* cfg, lbm_dense** out);
 lbm_status lbm_dense_destroy(lbm_dense* h);
 lbm_status lbm_dense_get_info(lbm_dense* h, lbm_dense_info* info);
 /* Canonical pre-collision state in / out (host arrays of q * ncell doubles);
- * set_state resets time_step to 0.  Solid cells read back as 0. */
+ * set_state resets time_step to 0.  Solid cells read back as 0.  The device
+ * storage order (x-rows plain or interleaved, chosen at create) is internal:
+ * every entry point takes and returns cell = x + nx * (y + ny * z) order. */
 lbm_status lbm_dense_set_state(lbm_dense* h, const double* f);
 lbm_status lbm_dense_get_state(lbm_dense* h, double* f);
 /* f = f^eq(rho, u) per cell: rho (ncell, NULL -> 1) and u (3 * ncell, xyz per cell)
