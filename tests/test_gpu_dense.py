@@ -609,11 +609,12 @@ IROWS = [(q, st, nx) for q in (19, 27) for st in ("f64", "f32s", "f16s") for nx 
 def test_dense_interleaved_rows(q, storage, nx, nt, monkeypatch):
     """AA on interleaved rows (element x of a row at position
     (x mod K) * 64 + x / K, K = nx / 64: the default on unmasked periodic SOA
-    boxes whose rows are whole 64-blocks) for K = 1, 3 and 4, with body force
-    and per-cell tau and gamma fields, cached and streaming: after 7 steps
-    (and a get/set round trip in the odd convention after 5), the state equals
-    the plain-row run bit for bit in every storage, and FP64 equals the oracle
-    bit for bit.  Moments are those of the plain-row run too."""
+    boxes whose rows are whole 64-blocks) for K = 1, 3 and 4, with body force,
+    with per-cell tau and gamma fields or with a pressure jump across x,
+    cached and streaming: after 8 steps (and a get/set round trip in the odd
+    convention after 5), the state equals the plain-row run bit for bit in
+    every storage, and FP64 equals the oracle bit for bit.  Moments are those
+    of the plain-row run too."""
     monkeypatch.setenv("LBM_DENSE_NT", nt)
     ny, nz = 6, 5
     geom = O.DenseGeometry(nx, ny, nz)
@@ -623,26 +624,31 @@ def test_dense_interleaved_rows(q, storage, nx, nt, monkeypatch):
     gamma = 0.3 * O.uniform01(62, cells)
     omega, F = 1 / 0.6, (1e-5, -2e-6, 3e-6)
     got = {}
+    variants = ("plain", "fields", "jump")
     for irows in ("0", "1"):
         monkeypatch.setenv("LBM_DENSE_IROWS", irows)
         print(f"[dense] interleaved rows={irows} q={q} {storage} nx={nx} nt={nt}", flush=True)
-        for fields in (False, True):
+        for fields in variants:
             with DenseLattice(nx, ny, nz, q=q, pattern="aa", storage=storage, omega=omega, force=F,
-                              tau=tau if fields else None, gamma=gamma if fields else None) as lat:
+                              tau=tau if fields == "fields" else None,
+                              gamma=gamma if fields == "fields" else None,
+                              density_jump_x=3e-3 if fields == "jump" else 0.0) as lat:
                 lat.set_state(fc)
                 lat.step(5)
                 lat.set_state(lat.get_state())  # t = 0 again: the even convention
                 lat.step(2)
                 lat.step(1)
                 got[irows, fields] = (lat.get_state(), lat.moments())
-    for fields in (False, True):
+    for fields in variants:
         a, b = got["0", fields], got["1", fields]
         assert np.array_equal(a[0], b[0]), fields
         assert np.array_equal(a[1][0], b[1][0]) and np.array_equal(a[1][1], b[1][1]), fields
     if storage == "f64":
-        assert np.array_equal(got["1", False][0], oracle_run(geom, q, fc, "aa", 8, omega, F))
-        assert np.array_equal(got["1", True][0],
+        assert np.array_equal(got["1", "plain"][0], oracle_run(geom, q, fc, "aa", 8, omega, F))
+        assert np.array_equal(got["1", "fields"][0],
                               oracle_run(geom, q, fc, "aa", 8, omega, F, tau=tau, gamma=gamma))
+        assert np.array_equal(got["1", "jump"][0],
+                              oracle_run_jump(geom, q, fc, "aa", 8, omega, F, 3e-3))
 
 
 @pytest.mark.parametrize("storage", ["f64", "f16s"])
