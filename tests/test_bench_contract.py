@@ -1,5 +1,6 @@
 """bench.py's command-line contract: one GPU only, and one JSON line carrying
-the fields the measurement needs (metric, value, roofline, dtype, config ...)."""
+the fields the measurement needs (metric, value, roofline, dtype, config, the
+copy-kernel ceiling ...)."""
 import json
 import os
 import subprocess
@@ -39,3 +40,7 @@ def test_small_run_prints_one_complete_json_line():
     assert line["config"]["workload"].startswith("D3Q19 AA SOA 64^3")
     f16 = line["f16s_storage"]
     assert f16["steps_compared"] >= 4 and 0 < f16["rel_l2_u_vs_f64"] < 1e-3
+    copy = line["copy_kernel"]
+    assert copy["unit"] == "GB/s" and 0 < copy["value"] < 8000.0
+    assert abs(copy["frac"] - copy["value"] / 8000.0) < 1e-3
+    assert abs(copy["frac_reached"] - roof["achieved"] / copy["value"]) < 1e-3
