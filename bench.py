@@ -16,8 +16,9 @@ by device synchronisation on both sides.  roofline.achieved = 2 Q b bytes per
 node * nodes / mean kernel duration from HIP events recorded on the engine's
 stream around the same K launches.
 
-copy_kernel: the measured copy ceiling of the same GPU (a 16-byte-per-lane
-copy of 4 GiB), and the fraction of it the headline leg reaches.
+copy_kernel: the measured copy ceiling of the same GPU (a non-temporal
+16-byte-per-lane copy of 4 GiB), and the fraction of it the headline leg
+reaches.
 
 cpu_baseline: the reference's own AA kernel (KernelAA_SOA_Unroll, the
 `unroll 1` kernel of its benchmark decks) compiled from its sources into
@@ -137,15 +138,16 @@ def adjoint_leg(n=128, steps=500, every=50, storage="f64"):
 
 
 def copy_peak(nbytes=4 << 30, reps=10):
-    """The measured copy ceiling of this GPU: the library's 16-byte copy kernel
-    between two nbytes buffers (nbytes read + nbytes written per dispatch,
-    reps timed dispatches after one untimed)."""
+    """The measured copy ceiling of this GPU: the library's non-temporal
+    16-byte copy kernel between two nbytes buffers (nbytes read + nbytes
+    written per dispatch, reps timed dispatches after one untimed)."""
     from lbmminiapp_amd import capi
 
     gbs = capi.copy_bandwidth(nbytes, reps)
     return {"value": round(gbs, 1), "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 4),
             "bytes_per_dispatch": 2 * nbytes,
-            "note": "16-byte-per-lane copy kernel, 4 GiB read + 4 GiB written per dispatch; "
+            "note": "non-temporal 16-byte-per-lane copy kernel, 4 GiB read + 4 GiB written per "
+                    "dispatch; "
                     "frac_reached = the headline leg's achieved bandwidth / this"}
 
 
