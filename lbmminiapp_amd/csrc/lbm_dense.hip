@@ -344,7 +344,10 @@ size_t lds_for_blocks_per_cu(int k) {
 //                averaged over 100-step runs with the interleaved-row odd
 //                step (256 wide, no cap: 1.76-1.78; 128 wide: 1.84-1.85)
 //   D3Q27 FP64   256 wide, <= 2 per CU: 9.31 ms (512-wide rows: 10.83 ms)
-//   D3Q27 FP32   128 wide, no cap:      4.69 ms (256 wide, no cap: 4.86 ms)
+//   D3Q27 FP32   256 wide, no cap: 4.92-4.95 ms per step, even and odd
+//                averaged over 100-step runs on plain rows, alternating with
+//                128 wide, no cap (5.01-5.07; in single launches right after
+//                initialisation 128 wide had ranked first, 4.69 vs 4.86 ms)
 //   D3Q27 FP16   512 wide, <= 4 per CU: 2.49-2.50 ms per step, even and odd
 //                averaged over 100-step runs with the interleaved-row odd
 //                step (128 wide, <= 8 per CU: 2.61-2.67; 256 / 4: 2.53)
@@ -366,7 +369,7 @@ void set_launch_shapes(lbm_dense* h) {
     static const int shape[2][3][2] = {
         // {width, workgroups per CU} for F64, F32S, F16S
         {{128, 4}, {256, 4}, {512, 4}},  // D3Q19
-        {{256, 2}, {128, 0}, {512, 4}},  // D3Q27
+        {{256, 2}, {256, 0}, {512, 4}},  // D3Q27
     };
     const int* sh = shape[h->q == 19 ? 0 : 1][h->storage];
     even_bx = sh[0];
