@@ -343,7 +343,9 @@ size_t lds_for_blocks_per_cu(int k) {
 //   D3Q19 FP16   512 wide, <= 4 per CU: 1.74 ms per step, even and odd
 //                averaged over 100-step runs with the interleaved-row odd
 //                step (256 wide, no cap: 1.76-1.78; 128 wide: 1.84-1.85)
-//   D3Q27 FP64   256 wide, <= 2 per CU: 9.31 ms (512-wide rows: 10.83 ms)
+//   D3Q27 FP64   128 wide, <= 2 per CU: 9.75-9.83 ms alternating with 256 wide,
+//                <= 2 per CU (10.30-10.33; step pairs 10.67-10.94 vs 10.86-11.28
+//                ms per step); 512-wide rows: 10.83 ms
 //   D3Q27 FP32   256 wide, no cap: 4.92-4.95 ms per step, even and odd
 //                averaged over 100-step runs on plain rows, alternating with
 //                128 wide, no cap (5.01-5.07; in single launches right after
@@ -369,7 +371,7 @@ void set_launch_shapes(lbm_dense* h) {
     static const int shape[2][3][2] = {
         // {width, workgroups per CU} for F64, F32S, F16S
         {{128, 4}, {256, 4}, {512, 4}},  // D3Q19
-        {{256, 2}, {256, 0}, {512, 4}},  // D3Q27
+        {{128, 2}, {256, 0}, {512, 4}},  // D3Q27
     };
     const int* sh = shape[h->q == 19 ? 0 : 1][h->storage];
     even_bx = sh[0];
