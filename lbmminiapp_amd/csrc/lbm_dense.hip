@@ -338,7 +338,10 @@ size_t lds_for_blocks_per_cu(int k) {
 // small workgroups at low occupancy, like the 19-stream in-place copy pattern,
 // whose DRAM efficiency drops as more streams are open per CU.  512^3 even
 // step, best of a sweep over widths 64-512 and 0-8 workgroups per CU:
-//   D3Q19 FP64   128 wide, <= 4 per CU: 6.91 ms (512-wide rows: 7.86 ms)
+//   D3Q19 FP64   128 wide, <= 3 per CU: 7.09-7.12 ms per step, even and odd
+//                averaged over 100-step runs alternating with <= 4 per CU
+//                (7.16-7.20), <= 5 (7.15-7.20) and <= 2 (7.09-7.13); single
+//                even steps 6.91 ms (512-wide rows: 7.86 ms)
 //   D3Q19 FP32   256 wide, <= 4 per CU: 3.39 ms (256 wide, no cap: 3.47 ms)
 //   D3Q19 FP16   512 wide, <= 4 per CU: 1.74 ms per step, even and odd
 //                averaged over 100-step runs with the interleaved-row odd
@@ -353,8 +356,8 @@ size_t lds_for_blocks_per_cu(int k) {
 //   D3Q27 FP16   512 wide, <= 4 per CU: 2.49-2.50 ms per step, even and odd
 //                averaged over 100-step runs with the interleaved-row odd
 //                step (128 wide, <= 8 per CU: 2.61-2.67; 256 / 4: 2.53)
-// FP64 checked again in 100-step runs: 128 wide, <= 4 per CU 7.00-7.08 ms per
-// step; 128 / 2: 7.05-7.09; 64 / 4: 7.10-7.13; wider or more per CU slower.
+// The first handle of a process measured faster than later ones, so shape
+// comparisons alternate the candidates, none of them first.
 // Environment overrides (tuning): LBM_DENSE_BLOCK_X and LBM_DENSE_BLOCKS_PER_CU
 // for the unstaged kernels (the latter also for the staged ones),
 // LBM_DENSE_EVEN_BLOCK_X and LBM_DENSE_EVEN_BLOCKS_PER_CU for the even step.
@@ -370,7 +373,7 @@ void set_launch_shapes(lbm_dense* h) {
   if (h->layout == LBM_LAYOUT_SOA) {
     static const int shape[2][3][2] = {
         // {width, workgroups per CU} for F64, F32S, F16S
-        {{128, 4}, {256, 4}, {512, 4}},  // D3Q19
+        {{128, 3}, {256, 4}, {512, 4}},  // D3Q19
         {{128, 2}, {256, 0}, {512, 4}},  // D3Q27
     };
     const int* sh = shape[h->q == 19 ? 0 : 1][h->storage];
