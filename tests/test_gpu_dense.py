@@ -3,9 +3,9 @@
 FP64 storage is the reference arithmetic: every pattern (AA, AB push, AB pull),
 layout (SOA, AOS) and lattice (D3Q19, D3Q27), with and without solid cells and
 non-periodic faces, equals the oracle bit for bit after an odd number of steps
-(so both AA readout conventions are exercised).  The north-star case, D3Q19 AA
-64^3 periodic Taylor-Green (tau = 0.6, F = 1e-6 x, 1000 steps), must agree to
-1e-12 relative per population and conserve mass to round-off.  FP32/FP16
+(so both AA readout conventions are exercised).  The D3Q19 AA 64^3 periodic
+Taylor-Green run (tau = 0.6, F = 1e-6 x, 1000 steps) must agree to 1e-12
+relative per population and conserve mass to round-off.  FP32/FP16
 deviation storage is checked against the FP64 run with stated tolerances."""
 import numpy as np
 import pytest
