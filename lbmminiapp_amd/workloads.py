@@ -120,7 +120,7 @@ def _time_dense(lat, steps, warmup):
 
 
 def tgv64(args):
-    """Configuration 1: D3Q19 AA SOA FP64 64^3 periodic Taylor-Green (u0 = 0.05 plus seeded
+    """D3Q19 AA SOA FP64 64^3 periodic Taylor-Green (u0 = 0.05 plus seeded
     U(-1e-4, 1e-4) noise, tau = 0.6), F = 0, 1000 steps."""
     n = 64
     steps = args.steps or 1000

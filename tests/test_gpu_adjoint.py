@@ -82,13 +82,13 @@ def test_checkpoint_interval_does_not_change_gradients():
         assert np.array_equal(gg, res[0][0]) and np.array_equal(gt, res[0][1])
 
 
-def config4_case(n=128):
+def brinkman_channel_case(n=128):
     solid = capi.channel(n, n, n)
     gamma = capi.smooth_random_field(n, n, n, 4.0, 4)
     return solid, gamma
 
 
-def test_config4_gradient_vs_finite_differences():
+def test_channel_128_brinkman_gradient_vs_finite_differences():
     """D3Q19 128^3 channel (walls y = 0, 127), gamma from seeded white noise
     (seed 4) smoothed with a periodic Gaussian (sigma = 4) and rescaled to
     [0, 1], F = (1e-6, 0, 0), tau = 0.8, J = kinetic energy after 500 steps,
@@ -99,8 +99,8 @@ def test_config4_gradient_vs_finite_differences():
     2e-2, while 4th- and 6th-order differences agree to 1e-7 at h = 4e-2
     (truncation negligible); measured with the CPU oracle at this size."""
     n, steps = 128, 500
-    solid, gamma = config4_case(n)
-    print("[adjoint] config4 128^3 forward/backward", flush=True)
+    solid, gamma = brinkman_channel_case(n)
+    print("[adjoint] Brinkman channel 128^3 forward/backward", flush=True)
     with Adjoint(n, n, n, q=19, storage="f64", omega=1 / 0.8, force=(1e-6, 0, 0), solid=solid,
                  gamma=gamma, objective="kinetic_energy", checkpoint_every=50) as adj:
         J = adj.forward(steps)
@@ -121,18 +121,18 @@ def test_config4_gradient_vs_finite_differences():
     assert worst <= 1e-6
 
 
-def test_config4_fp32_adjoint_vs_fp64():
-    """Config 4 at its full extent in FP32 deviation storage (FP32 forward and
+def test_channel_128_brinkman_fp32_adjoint_vs_fp64():
+    """The Brinkman channel case at its full extent in FP32 deviation storage (FP32 forward and
     adjoint arithmetic, FP64 gradient accumulation): D3Q19 128^3 channel,
     smooth gamma field, kinetic energy after 500 steps, checkpoint every 50.
     J within 1e-7 relative and the gamma gradient within 1e-6 rel-L2 of the
     FP64 adjoint (itself pinned to finite differences at 1e-6); measured on
     the MI355X: J 1.2e-8, gradient 3.4e-7."""
     n, steps = 128, 500
-    solid, gamma = config4_case(n)
+    solid, gamma = brinkman_channel_case(n)
     out = {}
     for st in ("f64", "f32s"):
-        print(f"[adjoint] config4 128^3 {st}", flush=True)
+        print(f"[adjoint] Brinkman channel 128^3 {st}", flush=True)
         with Adjoint(n, n, n, q=19, storage=st, omega=1 / 0.8, force=(1e-6, 0, 0), solid=solid,
                      gamma=gamma, checkpoint_every=50) as adj:
             J = adj.forward(steps)

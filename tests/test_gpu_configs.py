@@ -12,7 +12,7 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 
-def test_config2_channel_256_full_size():
+def test_channel_256_spheres_pull_full_size():
     """D3Q19 AB pull FP64, 256^3 channel (walls y = 0, 255), 5% spheres of
     radius 4-12 (seed 2), F = 1e-6 x, tau = 0.8: equal to the oracle bit for
     bit after 4 steps from a seeded state; then mass conserved to 1e-12 and
@@ -27,7 +27,7 @@ def test_config2_channel_256_full_size():
     fc = O.equilibrium(q, rho, u)
     del rho, u, cell
     omega, F = 1 / 0.8, (1e-6, 0.0, 0.0)
-    print("[config2] 256^3 dense vs oracle, 4 steps", flush=True)
+    print("[channel] 256^3 dense vs oracle, 4 steps", flush=True)
     full = np.zeros((q, geom.ncell))
     full[:, geom.cell_of_fluid] = fc
     with DenseLattice(n, n, n, q=q, pattern="ab", pull=True, storage="f64", periodic=(1, 0, 1),
@@ -84,7 +84,7 @@ def _precision_vs_f64(n, q, tau, steps, noise, gates):
         del rho, u
 
 
-def test_config3_d3q27_512_reduced_precision_2000_steps():
+def test_d3q27_512_reduced_precision_2000_steps():
     """D3Q27 AA 512^3 periodic Taylor-Green (u0 = 0.05, tau = 0.51, no noise, F = 0)
     over its 2000 steps: FP32 and FP16 deviation storage (FP32 collision, FP64
     moment sums) against the FP64 run, field by field.  Gates: velocity rel-L2
@@ -114,7 +114,7 @@ def _porous_384():
     return n, solid
 
 
-def test_config5_porous_384_aos_and_soa_3000_steps():
+def test_porous_384_aos_and_soa_3000_steps():
     """D3Q19 AB push FP32 deviations, 384^3 packed spheres (radius 6-16, 35%
     solid, seed 6), inlet/outlet pressure drop 1e-4 (density jump 3e-4 across
     the periodic x boundary), its 3000 steps, in the SOA and in the AOS layout
@@ -127,7 +127,7 @@ def test_config5_porous_384_aos_and_soa_3000_steps():
     x = np.arange(n ** 3) % n
     fields = {}
     for layout in ("soa", "aos"):
-        print(f"[config5] 384^3 porous {layout} x 3000", flush=True)
+        print(f"[porous] 384^3 porous {layout} x 3000", flush=True)
         with DenseLattice(n, n, n, q=19, pattern="ab", layout=layout, storage="f32s", omega=1.0,
                           solid=solid, density_jump_x=3e-4) as lat:
             lat.init_tgv(0.0, 0.0, 1)
@@ -136,7 +136,7 @@ def test_config5_porous_384_aos_and_soa_3000_steps():
             m1 = lat.mass()
             rho, vel = lat.moments()
         drho = rho[fluid & (x == 0)].mean() - rho[fluid & (x == n - 1)].mean()
-        print(f"[config5] {layout}: inlet - outlet {drho:.4e}, mean ux {vel[fluid, 0].mean():.3e}, "
+        print(f"[porous] {layout}: inlet - outlet {drho:.4e}, mean ux {vel[fluid, 0].mean():.3e}, "
               f"mass drift {(m1 - m0) / m0:.2e}", flush=True)
         assert abs(drho - 3e-4) < 0.05 * 3e-4, (layout, drho)
         assert vel[fluid, 0].mean() > 0
@@ -146,12 +146,12 @@ def test_config5_porous_384_aos_and_soa_3000_steps():
     (rs, us), (ra, ua) = fields["soa"], fields["aos"]
     e_u = np.linalg.norm(ua - us) / np.linalg.norm(us)
     e_r = np.abs(ra - rs).max()
-    print(f"[config5] AOS vs SOA: u rel-L2 {e_u:.3e}, max |drho| {e_r:.3e}, "
+    print(f"[porous] AOS vs SOA: u rel-L2 {e_u:.3e}, max |drho| {e_r:.3e}, "
           f"bitwise {np.array_equal(ua, us) and np.array_equal(ra, rs)}", flush=True)
     assert e_u <= 1e-6 and e_r <= 1e-7
 
 
-def test_config5_tau_adjoint_384_nested_checkpoints():
+def test_porous_384_tau_adjoint_nested_checkpoints():
     """Adjoint of the outlet mass flux (plane x = nx - 1) with respect to a
     per-node tau field in [0.55, 1.0] (smooth seeded field, sigma 4, seed 7)
     on the 384^3 porous medium with the pressure drop, nested checkpoints
@@ -179,26 +179,26 @@ def test_config5_tau_adjoint_384_nested_checkpoints():
     picks = near[(O.uniform01(12, np.arange(2, dtype=np.uint64)) * near.size).astype(int)]
     grads, objs = {}, {}
     for st in ("f64", "f32s"):
-        print(f"[config5-adjoint] 384^3 {st} x {steps}", flush=True)
+        print(f"[porous-adjoint] 384^3 {st} x {steps}", flush=True)
         with Adjoint(n, n, n, q=19, storage=st, omega=1.0, solid=solid, tau=tau,
                      objective="outlet_flux", outlet_x=n - 1, checkpoint_every=250,
                      checkpoint_inner=(25, 5), density_jump_x=3e-4) as adj:
             J = adj.forward(steps)
             _, gt, ms = adj.backward(gamma=False)
-            print(f"[config5-adjoint] {st}: J = {J:.6e}, backward {ms:.0f} ms, "
+            print(f"[porous-adjoint] {st}: J = {J:.6e}, backward {ms:.0f} ms, "
                   f"{fluid.size * steps / (ms * 1e3):.0f} adjoint MLUPS, "
                   f"{adj.device_bytes() / 1e9:.1f} GB", flush=True)
             if st == "f64":
                 pred = float(np.dot(gt[fluid], v[fluid]))
                 d = adj.finite_difference_direction("tau", v, 1e-3, steps, order=4)
                 rel = abs(pred - d) / abs(d)
-                print(f"[config5-adjoint] direction: adjoint {pred:.12e} fd {d:.12e} "
+                print(f"[porous-adjoint] direction: adjoint {pred:.12e} fd {d:.12e} "
                       f"rel {rel:.2e}", flush=True)
                 assert rel <= 1e-6, (pred, d)
                 for cell in picks:
                     d = adj.finite_difference("tau", int(cell), 1e-2, steps, order=6)
                     rel = abs(gt[cell] - d) / abs(d)
-                    print(f"[config5-adjoint] cell {cell}: adjoint {gt[cell]:.9e} fd {d:.9e} "
+                    print(f"[porous-adjoint] cell {cell}: adjoint {gt[cell]:.9e} fd {d:.9e} "
                           f"rel {rel:.2e}", flush=True)
                     assert rel <= 1e-4, (cell, gt[cell], d)
         grads[st] = gt[fluid]
@@ -206,5 +206,5 @@ def test_config5_tau_adjoint_384_nested_checkpoints():
         del gt
     e = np.linalg.norm(grads["f32s"] - grads["f64"]) / np.linalg.norm(grads["f64"])
     ej = abs(objs["f32s"] - objs["f64"]) / abs(objs["f64"])
-    print(f"[config5-adjoint] FP32 vs FP64: gradient rel-L2 {e:.3e}, J rel {ej:.3e}", flush=True)
+    print(f"[porous-adjoint] FP32 vs FP64: gradient rel-L2 {e:.3e}, J rel {ej:.3e}", flush=True)
     assert e <= 5e-2 and ej <= 1e-5

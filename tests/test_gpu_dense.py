@@ -354,7 +354,7 @@ def test_dense_staged_wide_rows_d3q27_f64(pattern, monkeypatch):
     assert np.array_equal(got, oracle_run(geom, 27, fc, pattern, 5, omega, F))
 
 
-def test_dense_config1_tgv_64cubed_1000_steps():
+def test_dense_tgv_64cubed_1000_steps():
     """D3Q19 BGK, AA, SOA, FP64, 64^3 fully periodic, tau = 0.6, rho0 = 1,
     Taylor-Green u0 = 0.05 + seeded U(-1e-4, 1e-4) noise, 1000 steps, with the
     reference's body force (1e-6, 0, 0): per-population relative error <= 1e-12
@@ -364,7 +364,7 @@ def test_dense_config1_tgv_64cubed_1000_steps():
     u = O.tgv_velocity(geom, u0=0.05, noise=1e-4, seed=1)
     fc = O.equilibrium(q, np.ones(geom.nfluid), u)
     omega, force, steps = 1 / 0.6, (1e-6, 0.0, 0.0), 1000
-    print("[dense] config1 64^3 x 1000 steps", flush=True)
+    print("[dense] TGV 64^3 x 1000 steps", flush=True)
     with DenseLattice(n, n, n, q=q, pattern="aa", storage="f64", omega=omega, force=force) as lat:
         lat.set_state(fc)
         m0 = lat.mass()
