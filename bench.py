@@ -284,7 +284,7 @@ def load_traffic(workload_key):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py: this build runs on exactly one GPU; the whole domain is one GPU's "
+        print("bench.py: the engine runs on exactly one GPU; the whole domain is one GPU's "
               f"subdomain (multi-GPU halo exchange is a design, not a measured path). "
               f"--gpus {args.gpus} is not supported.", file=sys.stderr)
         sys.exit(2)
