@@ -18,8 +18,8 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 
-def small_channel(q, seed=3):
-    nx, ny, nz = 16, 9, 6
+def small_channel(q, seed=3, nx=16):
+    ny, nz = 9, 6
     solid = np.zeros((nz, ny, nx), np.uint8)
     solid[:, 0, :] = 1
     solid[:, ny - 1, :] = 1
@@ -59,6 +59,29 @@ def test_adjoint_matches_oracle(q, objective):
     for got, want in ((gg[g.cell_of_fluid], ggo), (gt[g.cell_of_fluid], gto)):
         assert np.abs(got - want).max() <= 1e-10 * np.abs(want).max()
     assert not gg[solid != 0].any() and not gt[solid != 0].any()
+
+
+def test_adjoint_keeps_plain_rows(monkeypatch):
+    """The adjoint's kernels index its arrays by cell, so its engine keeps plain
+    x-rows even when interleaved rows are forced (LBM_DENSE_IROWS=1) on a box
+    whose rows are a multiple of 64: objective, final state and gradients
+    still equal the oracle adjoint."""
+    monkeypatch.setenv("LBM_DENSE_IROWS", "1")
+    q = 19
+    g, solid, gamma, tau, fc, full = small_channel(q, seed=5, nx=64)
+    steps, F = 8, (2e-5, 0.0, 0.0)
+    with Adjoint(g.nx, g.ny, g.nz, q=q, storage="f64", omega=1.0, force=F, solid=solid, tau=tau,
+                 gamma=gamma, objective=0, checkpoint_every=3) as adj:
+        adj.set_initial_state(full)
+        J = adj.forward(steps)
+        fT = adj.final_state()
+        gg, gt, _ = adj.backward()
+    Jo, ggo, gto, ffo = O.adjoint(q, g, fc, steps, 1.0, F, tau_node=tau[g.cell_of_fluid],
+                                  gamma_node=gamma[g.cell_of_fluid], objective=0)
+    assert np.array_equal(fT[:, g.cell_of_fluid], ffo)
+    assert abs(J - Jo) <= 1e-13 * abs(Jo)
+    for got, want in ((gg[g.cell_of_fluid], ggo), (gt[g.cell_of_fluid], gto)):
+        assert np.abs(got - want).max() <= 1e-10 * np.abs(want).max()
 
 
 def test_checkpoint_interval_does_not_change_gradients():
